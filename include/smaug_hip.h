@@ -1,0 +1,154 @@
+/*
+ * smaug_hip.h -- C-ABI of the MI355X-native SMAUG MHD time-step engine.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * SMAUG_OK (0) or a negative SMAUG_E* code; smaug_last_error() returns a
+ * human-readable message for the calling thread's last failure.
+ *
+ * Data contract (kept identical to the reference so states drop in):
+ *   - ghost width 2 per side; n[d] counts ghost cells (reference
+ *     src/iosmaug.c:47-57);
+ *   - field-major SoA, i (dim 0) fastest:
+ *       idx = f*N + (k*n1 + j)*n0 + i,   N = n0*n1*n2
+ *     (reference src/gradops.cu:89-97, fencode3/encode3);
+ *   - field order of the state vector (reference include/iotypes.h:211,215):
+ *       2D (NVAR=10): rho mom1 mom2 energy b1 b2 energyb rhob b1b b2b
+ *       3D (NVAR=13): rho mom1 mom2 mom3 energy b1 b2 b3 energyb rhob b1b b2b b3b
+ *     The first 2+2*ndim fields are perturbations that evolve, the rest are
+ *     the static background;
+ *   - boundary codes (reference include/iotypes.h:144):
+ *       0 periodic, 1 mpi, 2 mpiperiod, 3 cont, 4 contcd4, 5 fixed,
+ *       6 symm, 7 asymm.  This build implements 0 and 4.
+ *
+ * Reference interfaces replaced (file:line in mikeg64/smaug):
+ *   smaug_create      <- cuinit (src/init.cu:2222) + initgrid (src/init.cu:2502)
+ *                        + initial cuboundary pass (src/iosmaug.c:480-488)
+ *   smaug_step        <- body of the time loop (src/iosmaug.c:496-1144):
+ *                        rkon=0 single stage (src/iosmaug.c:631-835) or the
+ *                        intended VAC four-step RK (src/advance.cu:76-99)
+ *   smaug_cfl         <- cucomputec + cucomputemaxc and the dt rule
+ *                        (src/computedervfields.cu:2234,1864; src/iosmaug.c:541-616)
+ *   smaug_fill_ghosts <- cuboundary (src/boundary.cu:231,
+ *                        models/boundary.cu.fluxtube1:1137)
+ *   smaug_get_state   <- cuupdate device->host copy (src/update.cu:165-236)
+ *   smaug_set_state   <- host->device state upload in cuinit (src/init.cu:2222)
+ *   smaug_get_grid    <- the wd positions/delx written by initgrid
+ *                        (src/init.cu:2926-3010)
+ *   smaug_destroy     <- cufinish (src/update.cu:477)
+ */
+#ifndef SMAUG_HIP_H_
+#define SMAUG_HIP_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMAUG_OK 0
+#define SMAUG_E_ARG -1      /* invalid argument or parameter set         */
+#define SMAUG_E_HIP -2      /* a HIP runtime call failed                 */
+#define SMAUG_E_NOMEM -3    /* allocation failed                         */
+#define SMAUG_E_NUMERIC -4  /* non-finite wave speed / state (blow-up)   */
+#define SMAUG_E_UNSUPPORTED -5
+
+#define SMAUG_MAXVAR 13
+#define SMAUG_MAXDIM 3
+
+/* term-group mask: which parts of the right-hand side a step applies.
+ * SMAUG_TERMS_ALL reproduces the reference step; the others exist so each
+ * kernel family can be checked on its own. */
+#define SMAUG_T_FLUX 0x01      /* centdiff1/centdiff2 flux divergence      */
+#define SMAUG_T_ETERM 0x02     /* background energy terms (centdiff2c)     */
+#define SMAUG_T_GRAV 0x04      /* gravity (cugrav)                         */
+#define SMAUG_T_HRHO 0x08      /* density hyperdiffusion                   */
+#define SMAUG_T_HE 0x10        /* energy hyperdiffusion                    */
+#define SMAUG_T_HMOM 0x20      /* momentum hyperdiffusion (diag + cross)   */
+#define SMAUG_T_HB 0x40        /* magnetic hyperdiffusion (diag + cross)   */
+#define SMAUG_T_SOURCE 0x80    /* user source (driver) plug-in             */
+#define SMAUG_TERMS_ALL 0xFF
+
+/* user-source plug-ins (reference models/usersource.cu.*) */
+#define SMAUG_DRIVER_NONE 0       /* usersource.cu.default                 */
+#define SMAUG_DRIVER_VERTICAL 1   /* Gaussian vertical-velocity driver     */
+
+/* arithmetic of the evolved state */
+#define SMAUG_PREC_FP64 0
+#define SMAUG_PREC_MIXED 1        /* FP32 stencil shadows, FP64 master     */
+
+typedef struct smaug_params {
+  int ndim;                 /* 2 or 3                                        */
+  int n[3];                 /* cells per dim including 2+2 ghosts (n[2]=1 in 2D) */
+  double gamma;             /* PGAMMA                                        */
+  double mu;                /* PMU                                           */
+  double eta;               /* PETA                                          */
+  double g[3];              /* PGRAV0..2                                     */
+  double dt;                /* DT: fixed step, or the cap when moddton=1     */
+  double courant;           /* PCOURANT                                      */
+  int rkon;                 /* PRKON: 0 single stage, 1 four-step RK         */
+  int moddton;              /* PMODDTON: 1 = CFL step from the wave speeds   */
+  int hyperdifmom;          /* PHYPERDIFMOM                                  */
+  int divbon;               /* PDIVBON (must be 0 here)                      */
+  double chyp[SMAUG_MAXVAR];/* PCHYP* per field                              */
+  double xmin[3], xmax[3];  /* domain (interior)                             */
+  double dx[3];             /* p->dx = (xmax-xmin)/NI, used by the CFL rule  */
+  int boundtype[SMAUG_MAXVAR][3][2]; /* [field][dir][0=upper BCU,1=lower BCL] */
+  int driver;               /* SMAUG_DRIVER_*                                */
+  double drv[8];            /* driver parameters             */
+  unsigned terms;           /* SMAUG_T_* mask, SMAUG_TERMS_ALL normally      */
+  int precision;            /* SMAUG_PREC_*                                  */
+  double qt0;               /* initial value of the nominal clock p->qt      */
+  int it0;                  /* initial step counter p->it                    */
+} smaug_params;
+
+typedef struct smaug_ctx smaug_t;
+
+/* Allocates the device state, uploads the initial state w0 (NVAR*N doubles,
+ * ghost cells included) and the cell positions pos (ndim*N doubles, the
+ * reference's wd pos1..pos3 slab), derives delx by the reference's grid
+ * extrapolation and fills the ghost cells of the perturbation fields. */
+int smaug_create(const smaug_params* p, const double* w0, const double* pos,
+                 smaug_t** out);
+
+/* Advances nsteps full time steps (rkon=0: one stage each; rkon=1: four). */
+int smaug_step(smaug_t* s, int nsteps);
+
+/* Wave-speed maxima of the current state per direction (max over all cells
+ * of max(cfast_dir, c_sound)) and the time step the CFL rule would pick. */
+int smaug_cfl(smaug_t* s, double* cmax_per_dim, double* dt);
+
+/* Re-applies the boundary conditions to the current perturbation fields. */
+int smaug_fill_ghosts(smaug_t* s);
+
+/* Copies the full current state (NVAR*N doubles) to / from host memory. */
+int smaug_get_state(const smaug_t* s, double* w_host);
+int smaug_set_state(smaug_t* s, const double* w_host);
+
+/* Copies positions (ndim*N) and delx (ndim*N) as used by the solver. */
+int smaug_get_grid(const smaug_t* s, double* pos_host, double* delx_host);
+
+/* Current step counter, simulated time (sum of dt), nominal clock qt and
+ * the dt of the last step. */
+int smaug_get_clock(const smaug_t* s, int* it, double* t, double* qt,
+                    double* dt_last);
+
+/* Device pointer of the current state (NVAR*N doubles, FP64) so a caller that
+ * owns the stream can read it in place; valid until the next smaug_step. */
+int smaug_state_device_ptr(const smaug_t* s, void** dptr);
+
+/* Blocks until all work queued by the context has finished. */
+int smaug_sync(smaug_t* s);
+
+int smaug_destroy(smaug_t* s);
+
+const char* smaug_last_error(void);
+const char* smaug_version(void);
+
+/* Kernel timing hooks for the benchmark: launches `reps` RK stages of the
+ * configured step on the current state and reports the mean device time of
+ * each kernel family measured with HIP events on the context's stream. */
+int smaug_time_kernels(smaug_t* s, int nsteps, double* ms_prim,
+                       double* ms_rhs, double* ms_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMAUG_HIP_H_ */

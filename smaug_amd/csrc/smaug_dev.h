@@ -1,0 +1,148 @@
+// smaug_dev.h -- geometry, index maps and stencil operators shared by the
+// MI355X kernels (device) and the host set-up code.
+//
+// Layout (identical to the reference, src/gradops.cu:89-97): field-major,
+// dim 0 fastest, ghost width 2.  Internally the evolved perturbation fields
+// and the static background are separate arrays:
+//   pert  : rho, mom[0..nd), energy, b[0..nd)         (np = 2+2*nd fields)
+//   bg    : energyb, rhob, bb[0..nd)                  (nb = 2+nd fields)
+//   prim  : v[0..nd), p_t, q_e                        (cell-local, per stage)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smaug {
+
+constexpr int kMaxPert = 8;
+constexpr int kMaxVar = 13;
+
+// run constants passed by value to every kernel
+struct Geo {
+  int nd;
+  int n[3];
+  long long N;
+  long long st[3];       // strides of dims 0,1,2
+  int np;                // evolved fields
+  double gamma;
+  double g[3];
+  double chyp[kMaxPert]; // hyperdiffusion coefficient per evolved field
+  unsigned zm[kMaxVar];  // bit d: boundtype[idx][d][0] != 0 (gradient zeroing)
+  int bct[kMaxPert][3];  // boundary type per evolved field and dir
+  int bc_uniform;        // 1 when every evolved field has the same types
+  unsigned terms;        // SMAUG_T_* mask
+  int hyper;             // PHYPERDIFMOM
+  int driver;
+  double drv[8];
+};
+
+// evolved-field indices
+__host__ __device__ inline int fRHO() { return 0; }
+__host__ __device__ inline int fMOM(int d) { return 1 + d; }
+__host__ __device__ inline int fE(int nd) { return 1 + nd; }
+__host__ __device__ inline int fB(int nd, int d) { return 2 + nd + d; }
+// background indices
+__host__ __device__ inline int bEB() { return 0; }
+__host__ __device__ inline int bRHOB() { return 1; }
+__host__ __device__ inline int bBB(int d) { return 2 + d; }
+// prim indices
+__host__ __device__ inline int pV(int d) { return d; }
+__host__ __device__ inline int pPT(int nd) { return nd; }
+__host__ __device__ inline int pQE(int nd) { return nd + 1; }
+
+// Boundary-table indices the reference's gradient calls pass as "field"
+// (quirk Q2: derived and temporary arrays index boundtype[] with their own
+// enum value).  Values of the DEV/TEV enums in include/iotypes.h:225-242.
+__host__ __device__ inline int zFLUX(int nd) { return nd; }  // DEV flux
+__host__ __device__ inline int zVEL(int d) { return d; }     // DEV vel1+d
+constexpr int zRHO = 0, zTMP1 = 0, zTMP3 = 2, zTMP4 = 3, zTMP5 = 4, zTMP7 = 6, zTMP8 = 7;
+
+// ---------------------------------------------------------------------------
+// boundary index maps (periodic: src/boundary.cu:132-136,209-213;
+// contcd4: models/boundary.cu.fluxtube1:59-64)
+// For ghost coordinate x returns the interior coordinate it copies.
+__host__ __device__ inline int bc_src(int type, int x, int n) {
+  if (type == 0) {  // periodic
+    if (x < 2) return x + n - 4;
+    if (x >= n - 2) return x - (n - 4);
+    return x;
+  }
+  // contcd4: 0<-4, 1<-3, n-1<-n-5, n-2<-n-4
+  if (x < 2) return 4 - x;
+  if (x == n - 1) return n - 5;
+  if (x == n - 2) return n - 4;
+  return x;
+}
+
+// Ghost coordinates whose value is copied from interior coordinate y
+// (at most two per dim); returns how many were written to out[].
+__host__ __device__ inline int bc_images(int type, int y, int n, int* out) {
+  int m = 0;
+  if (type == 0) {
+    if (y == n - 4) out[m++] = 0;
+    if (y == n - 3) out[m++] = 1;
+    if (y == 2) out[m++] = n - 2;
+    if (y == 3) out[m++] = n - 1;
+  } else {
+    if (y == 4) out[m++] = 0;
+    if (y == 3) out[m++] = 1;
+    if (y == n - 5) out[m++] = n - 1;
+    if (y == n - 4) out[m++] = n - 2;
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Right-chain hyperdiffusion halo (quirk Q7).  The reference writes q of
+// cell t into the padded (n+2)^d array at linear index t + C (unpadded
+// encoder on a padded array, src/hyperdifvisc1r.cu:1235,1253) in the same
+// launch that zeroes that array; with threads ordered by linear index the
+// stray value survives at padded cell H iff its writer t = L(H) - C is a
+// valid thread not earlier than H's zeroing thread.  Returns the unpadded
+// linear index of the source cell, or -1 for zero.
+__host__ __device__ inline long long halo_r_src(const Geo& G, const int* pc) {
+  const long long P0 = G.n[0] + 2, P1 = G.n[1] + 2;
+  long long LH, C, tz;
+  int z0 = pc[0] < G.n[0] - 1 ? pc[0] : G.n[0] - 1;
+  int z1 = pc[1] < G.n[1] - 1 ? pc[1] : G.n[1] - 1;
+  if (G.nd == 3) {
+    int z2 = pc[2] < G.n[2] - 1 ? pc[2] : G.n[2] - 1;
+    LH = ((long long)pc[2] * P1 + pc[1]) * P0 + pc[0];
+    C = (long long)G.n[0] * G.n[1] + G.n[0] + 1;
+    tz = ((long long)z2 * G.n[1] + z1) * G.n[0] + z0;
+  } else {
+    LH = (long long)pc[1] * P0 + pc[0];
+    C = G.n[0] + 1;
+    tz = (long long)z1 * G.n[0] + z0;
+  }
+  long long h = LH - C;
+  if (h < 0 || h >= G.N || h < tz) return -1;
+  return h;
+}
+
+struct RhsArgs {
+  const double* win;    // stage input, evolved fields
+  const double* bg;     // background
+  const double* prim;   // k_prim output of win
+  const double* delx;   // nd*N
+  const double* pos;    // nd*N
+  const double* base;   // stage base (w^n), read at the cell only
+  const double* r1;     // RK combination inputs (final stage only, else null)
+  const double* r2;
+  const double* r3;
+  double* wout;
+  const unsigned long long* cmax_bits;  // this stage's maxima
+  unsigned long long* zero_next;        // maxima slot of the next stage
+  const double* dts;                    // device dt
+  double hfrac;                         // stage step = hfrac*dt
+  double tdep;                          // driver time factor sin(2 pi qt/P)
+};
+
+void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg, double* prim,
+                 unsigned long long* cmax, int* flag);
+void launch_zero3(hipStream_t st, unsigned long long* p);
+void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,
+               double dtcap, int moddton, double* dts);
+void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
+void launch_fill(const Geo& G, hipStream_t st, double* w);
+
+}  // namespace smaug

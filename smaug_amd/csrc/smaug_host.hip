@@ -1,0 +1,496 @@
+// smaug_host.hip -- C-ABI of the engine (declared in include/smaug_hip.h):
+// device state, grid set-up, stage orchestration on one HIP stream.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/smaug_hip.h"
+#include "smaug_dev.h"
+
+
+using namespace smaug;
+
+static thread_local std::string g_err;
+
+#include <stdarg.h>
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return fail(SMAUG_E_HIP, "%s (code %d)", hipGetErrorString(e_), (int)e_); \
+  } while (0)
+
+struct smaug_ctx {
+  smaug_params p;
+  Geo G;
+  hipStream_t stream = nullptr;
+  int nd = 0, np = 0, nb = 0, npr = 0, nvar = 0;
+  long long N = 0;
+  double* U[4] = {nullptr, nullptr, nullptr, nullptr};  // evolved-field buffers
+  double* bg = nullptr;
+  double* prim = nullptr;
+  double* delx = nullptr;
+  double* pos = nullptr;
+  unsigned long long* cmax = nullptr;  // 3 slots of 4
+  double* dts = nullptr;               // dt, t
+  int* flag = nullptr;
+  int cur = 0;      // buffer holding the current state
+  int cslot = 0;    // maxima slot of the next k_prim
+  double qt = 0;
+  int it = 0;
+  std::vector<double> h_pos, h_delx;
+};
+
+// ---- grid: positions extrapolated past the interior, delx (initgrid,
+// src/init.cu:2502-3051, quirk Q11) -----------------------------------------
+static void initgrid_host(int nd, const int* n, const double* pos, std::vector<double>& opos,
+                          std::vector<double>& odelx) {
+  const long long n0 = n[0], n1 = n[1], n2 = nd == 3 ? n[2] : 1;
+  const long long N = n0 * n1 * n2;
+  const long long P0 = n0 + 2, P1 = n1 + 2, P2 = nd == 3 ? n2 + 2 : 1;
+  const long long NP2 = P0 * P1 * P2;
+  auto e2 = [&](long long a, long long b, long long c) { return (c * P1 + b) * P0 + a; };
+  std::vector<double> tt((size_t)3 * NP2, 0.0);
+  for (int d = 0; d < nd; d++)
+    for (long long k = 0; k < n2; k++)
+      for (long long j = 0; j < n1; j++)
+        for (long long i = 0; i < n0; i++)
+          tt[d * NP2 + e2(i + 1, j + 1, nd == 3 ? k + 1 : 0)] = pos[d * N + (k * n1 + j) * n0 + i];
+  const long long PP[3] = {P0, P1, P2};
+  for (int dir = 0; dir < nd; dir++) {
+    const int nn = n[dir];
+    for (int side = 0; side < 2; side++) {
+      int ixmin, ixmax, ixe, ixf;
+      if (side == 0) {
+        ixmax = nn + 1;
+        ixmin = nn - 1;
+        ixe = ixmin - 1;
+        ixf = ixe - 1;
+      } else {
+        ixmin = 0;
+        ixmax = dir == 0 ? 1 : 2;
+        ixe = ixmax + 1;
+        ixf = ixe + 1;
+      }
+      for (int d1 = 0; d1 < nd; d1++) {
+        double* T = tt.data() + d1 * NP2;
+        for (long long c2 = 0; c2 < PP[2]; c2++)
+          for (long long c1 = 0; c1 < PP[1]; c1++)
+            for (long long c0 = 0; c0 < PP[0]; c0++) {
+              long long cc[3] = {c0, c1, c2};
+              const int ix = (int)cc[dir];
+              if (ix < ixmin || ix > ixmax) continue;
+              long long e1[3] = {c0, c1, c2}, f1[3] = {c0, c1, c2};
+              e1[dir] = ixe;
+              f1[dir] = ixf;
+              const double a = (double)(1 + abs(ixe - ix)), b = (double)abs(ixe - ix);
+              T[e2(c0, c1, c2)] = a * (T[e2(e1[0], e1[1], e1[2])]) - b * (T[e2(f1[0], f1[1], f1[2])]);
+            }
+      }
+    }
+  }
+  opos.assign((size_t)nd * N, 0.0);
+  odelx.assign((size_t)nd * N, 0.0);
+  for (int d = 0; d < nd; d++) {
+    const double* T = tt.data() + d * NP2;
+    const long long o0 = d == 0, o1 = d == 1, o2 = d == 2;
+    for (long long k = 0; k < n2; k++)
+      for (long long j = 0; j < n1; j++)
+        for (long long i = 0; i < n0; i++) {
+          const long long c = (k * n1 + j) * n0 + i, kp = nd == 3 ? k + 1 : 0;
+          opos[d * N + c] = T[e2(i + 1, j + 1, kp)];
+          odelx[d * N + c] = 0.5 * (T[e2(i + 1 + o0, j + 1 + o1, kp + o2)] - T[e2(i + 1 - o0, j + 1 - o1, kp - o2)]);
+        }
+  }
+}
+
+static int check_params(const smaug_params* p) {
+  if (p->ndim != 2 && p->ndim != 3) return fail(SMAUG_E_ARG, "ndim must be 2 or 3");
+  for (int d = 0; d < p->ndim; d++)
+    if (p->n[d] < 8) return fail(SMAUG_E_ARG, "n[%d] must be >= 8 (got %d)", d, p->n[d]);
+  if (p->ndim == 2 && p->n[2] != 1) return fail(SMAUG_E_ARG, "2D runs need n[2] == 1");
+  if (p->divbon) return fail(SMAUG_E_UNSUPPORTED, "divbon=1 is not supported");
+  if (p->rkon != 0 && p->rkon != 1) return fail(SMAUG_E_ARG, "rkon must be 0 or 1");
+  if (p->precision != SMAUG_PREC_FP64 && p->precision != SMAUG_PREC_MIXED)
+    return fail(SMAUG_E_ARG, "unknown precision");
+  if (p->driver != SMAUG_DRIVER_NONE && p->driver != SMAUG_DRIVER_VERTICAL)
+    return fail(SMAUG_E_ARG, "unknown driver");
+  if (p->driver == SMAUG_DRIVER_VERTICAL && (p->drv[1] == 0 || p->drv[5] == 0 || p->drv[6] == 0))
+    return fail(SMAUG_E_ARG, "driver needs a period and widths (drv[1], drv[5], drv[6])");
+  const int np = 2 + 2 * p->ndim;
+  for (int f = 0; f < np; f++)
+    for (int d = 0; d < p->ndim; d++) {
+      const int bt = p->boundtype[f][d][0];
+      if (bt == 0 && p->ndim == 3)
+        return fail(SMAUG_E_UNSUPPORTED, "periodic boundaries are supported in 2D only");
+      if (bt != 0 && bt != 4)
+        return fail(SMAUG_E_UNSUPPORTED, "boundary type %d not supported (0 and 4 are)", bt);
+    }
+  if (p->moddton == 1 && !(p->courant > 0)) return fail(SMAUG_E_ARG, "courant must be > 0");
+  if (!(p->dt > 0)) return fail(SMAUG_E_ARG, "dt must be > 0");
+  return SMAUG_OK;
+}
+
+static void make_geo(const smaug_params* p, Geo& G) {
+  memset(&G, 0, sizeof G);
+  G.nd = p->ndim;
+  G.n[0] = p->n[0];
+  G.n[1] = p->n[1];
+  G.n[2] = p->ndim == 3 ? p->n[2] : 1;
+  G.N = (long long)G.n[0] * G.n[1] * G.n[2];
+  G.st[0] = 1;
+  G.st[1] = G.n[0];
+  G.st[2] = (long long)G.n[0] * G.n[1];
+  G.np = 2 + 2 * p->ndim;
+  G.gamma = p->gamma;
+  for (int d = 0; d < 3; d++) G.g[d] = p->g[d];
+  for (int f = 0; f < G.np; f++) G.chyp[f] = p->chyp[f];
+  for (int idx = 0; idx < kMaxVar; idx++) {
+    unsigned m = 0;
+    for (int d = 0; d < p->ndim; d++)
+      if (p->boundtype[idx][d][0] != 0) m |= 1u << d;
+    G.zm[idx] = m;
+  }
+  G.bc_uniform = 1;
+  for (int f = 0; f < G.np; f++)
+    for (int d = 0; d < 3; d++) {
+      G.bct[f][d] = d < p->ndim ? p->boundtype[f][d][0] : 4;
+      if (G.bct[f][d] != G.bct[0][d]) G.bc_uniform = 0;
+    }
+  G.terms = p->terms;
+  G.hyper = p->hyperdifmom;
+  G.driver = p->driver;
+  for (int i = 0; i < 8; i++) G.drv[i] = p->drv[i];
+}
+
+static void free_ctx(smaug_t* s) {
+  if (!s) return;
+  for (auto& u : s->U)
+    if (u) hipFree(u);
+  hipFree(s->bg);
+  hipFree(s->prim);
+  hipFree(s->delx);
+  hipFree(s->pos);
+  hipFree(s->cmax);
+  hipFree(s->dts);
+  hipFree(s->flag);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+extern "C" int smaug_create(const smaug_params* p, const double* w0, const double* pos, smaug_t** out) {
+  if (!p || !w0 || !pos || !out) return fail(SMAUG_E_ARG, "null argument");
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (p->precision != SMAUG_PREC_FP64)
+    return fail(SMAUG_E_UNSUPPORTED, "mixed precision is not built in this library version");
+  smaug_t* s = new smaug_t();
+  s->p = *p;
+  make_geo(p, s->G);
+  s->nd = p->ndim;
+  s->np = 2 + 2 * s->nd;
+  s->nb = 2 + s->nd;
+  s->npr = s->nd + 2;
+  s->nvar = 4 + 3 * s->nd;
+  s->N = s->G.N;
+  const size_t bytes = sizeof(double) * (size_t)s->N;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(SMAUG_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  const int nbuf = p->rkon == 1 ? 4 : 2;
+  bool ok = true;
+  for (int b = 0; b < nbuf; b++) ok &= hipMalloc(&s->U[b], bytes * s->np) == hipSuccess;
+  ok &= hipMalloc(&s->bg, bytes * s->nb) == hipSuccess;
+  ok &= hipMalloc(&s->prim, bytes * s->npr) == hipSuccess;
+  ok &= hipMalloc(&s->delx, bytes * s->nd) == hipSuccess;
+  ok &= hipMalloc(&s->pos, bytes * s->nd) == hipSuccess;
+  ok &= hipMalloc(&s->cmax, sizeof(unsigned long long) * 12) == hipSuccess;
+  ok &= hipMalloc(&s->dts, sizeof(double) * 2) == hipSuccess;
+  ok &= hipMalloc(&s->flag, sizeof(int)) == hipSuccess;
+  if (!ok) {
+    free_ctx(s);
+    return fail(SMAUG_E_NOMEM, "device allocation failed");
+  }
+  initgrid_host(s->nd, p->n, pos, s->h_pos, s->h_delx);
+  const double dts0[2] = {p->moddton == 1 ? 1.0e-8 : p->dt, 0.0};  // src/iosmaug.c:105,333-336
+  const unsigned long long z12[12] = {0};
+  const int z = 0;
+  if (hipMemcpy(s->U[0], w0, bytes * s->np, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->bg, w0 + (size_t)s->np * s->N, bytes * s->nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->delx, s->h_delx.data(), bytes * s->nd, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->pos, s->h_pos.data(), bytes * s->nd, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->cmax, z12, sizeof z12, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->dts, dts0, sizeof dts0, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->flag, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess) {
+    free_ctx(s);
+    return fail(SMAUG_E_HIP, "initial upload failed");
+  }
+  s->qt = p->qt0;
+  s->it = p->it0;
+  s->cur = 0;
+  launch_fill(s->G, s->stream, s->U[0]);  // src/iosmaug.c:480-488
+  e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    free_ctx(s);
+    return fail(SMAUG_E_HIP, "initial boundary fill: %s", hipGetErrorString(e));
+  }
+  *out = s;
+  return SMAUG_OK;
+}
+
+static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, double* wout, double hfrac,
+                        double tdep, int slot) {
+  RhsArgs A;
+  A.win = win;
+  A.bg = s->bg;
+  A.prim = s->prim;
+  A.delx = s->delx;
+  A.pos = s->pos;
+  A.base = base;
+  A.r1 = A.r2 = A.r3 = nullptr;
+  A.wout = wout;
+  A.cmax_bits = s->cmax + 4 * slot;
+  A.zero_next = s->cmax + 4 * (slot ^ 1);
+  A.dts = s->dts;
+  A.hfrac = hfrac;
+  A.tdep = tdep;
+  return A;
+}
+
+// one stage: prim of win into the current maxima slot, then the rhs
+static void run_stage(smaug_t* s, const double* win, const double* base, double* wout, double hfrac, double tdep,
+                      bool first, const double* r1, const double* r2, const double* r3) {
+  const int slot = s->cslot;
+  launch_prim(s->G, s->stream, win, s->bg, s->prim, s->cmax + 4 * slot, s->flag);
+  if (first)
+    launch_dt(s->stream, s->cmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->dts);
+  RhsArgs A = rhs_args(s, win, base, wout, hfrac, tdep, slot);
+  A.r1 = r1;
+  A.r2 = r2;
+  A.r3 = r3;
+  launch_rhs(s->G, s->stream, A);
+  s->cslot ^= 1;
+}
+
+static double driver_tdep(const smaug_t* s) {
+  if (s->p.driver != SMAUG_DRIVER_VERTICAL) return 0.0;
+  return sin(2.0 * M_PI * s->qt / s->p.drv[1]);
+}
+
+static int step_once(smaug_t* s) {
+  s->qt = s->qt + s->p.dt;  // p->qt += DT (src/iosmaug.c:636, quirk Q5)
+  const double tdep = driver_tdep(s);
+  if (s->p.rkon == 0) {
+    double* win = s->U[s->cur];
+    double* wout = s->U[s->cur ^ 1];
+    run_stage(s, win, win, wout, 1.0, tdep, true, nullptr, nullptr, nullptr);
+    s->cur ^= 1;
+  } else {
+    double *w0 = s->U[0], *w1 = s->U[1], *w2 = s->U[2], *w3 = s->U[3];
+    run_stage(s, w0, w0, w1, 0.5, tdep, true, nullptr, nullptr, nullptr);
+    run_stage(s, w1, w0, w2, 0.5, tdep, false, nullptr, nullptr, nullptr);
+    run_stage(s, w2, w0, w3, 1.0, tdep, false, nullptr, nullptr, nullptr);
+    run_stage(s, w3, w0, w0, 1.0 / 6.0, tdep, false, w1, w2, w3);
+  }
+  s->it = s->it + 1;
+  HIPCHK(hipGetLastError());
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_step(smaug_t* s, int nsteps) {
+  if (!s || nsteps < 0) return fail(SMAUG_E_ARG, "bad arguments to smaug_step");
+  for (int n = 0; n < nsteps; n++) {
+    int rc = step_once(s);
+    if (rc) return rc;
+  }
+  return SMAUG_OK;
+}
+
+static int check_flag(smaug_t* s) {
+  int f = 0;
+  HIPCHK(hipMemcpyAsync(&f, s->flag, sizeof f, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (f) return fail(SMAUG_E_NUMERIC, "non-finite wave speed in the state (blow-up)");
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_sync(smaug_t* s) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return check_flag(s);
+}
+
+extern "C" int smaug_cfl(smaug_t* s, double* cmax_per_dim, double* dt) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  unsigned long long* slot = s->cmax + 8;
+  launch_zero3(s->stream, slot);
+  launch_prim(s->G, s->stream, s->U[s->cur], s->bg, s->prim, slot, s->flag);
+  unsigned long long bits[3];
+  double dts[2];
+  HIPCHK(hipMemcpyAsync(bits, slot, sizeof bits, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(dts, s->dts, sizeof dts, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  double cm[3];
+  for (int d = 0; d < 3; d++) memcpy(&cm[d], &bits[d], sizeof(double));
+  if (cmax_per_dim)
+    for (int d = 0; d < s->nd; d++) cmax_per_dim[d] = cm[d];
+  if (dt) {
+    double v = dts[0];
+    if (s->p.moddton == 1) {
+      double courantmax = 1.0e-12;
+      for (int d = 0; d < s->nd; d++) {
+        double t = cm[s->nd - 1] / s->p.dx[d];
+        if (t > courantmax) courantmax = t;
+      }
+      double dtt = s->p.courant / courantmax;
+      if (dtt > 1.0e-12 && dtt < s->p.dt) v = dtt;
+    }
+    *dt = v;
+  }
+  return check_flag(s);
+}
+
+extern "C" int smaug_fill_ghosts(smaug_t* s) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  launch_fill(s->G, s->stream, s->U[s->cur]);
+  HIPCHK(hipGetLastError());
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_get_state(const smaug_t* cs, double* w) {
+  smaug_t* s = const_cast<smaug_t*>(cs);
+  if (!s || !w) return fail(SMAUG_E_ARG, "null argument");
+  const size_t bytes = sizeof(double) * (size_t)s->N;
+  HIPCHK(hipMemcpyAsync(w, s->U[s->cur], bytes * s->np, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(w + (size_t)s->np * s->N, s->bg, bytes * s->nb, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return check_flag(s);
+}
+
+extern "C" int smaug_set_state(smaug_t* s, const double* w) {
+  if (!s || !w) return fail(SMAUG_E_ARG, "null argument");
+  const size_t bytes = sizeof(double) * (size_t)s->N;
+  HIPCHK(hipMemcpyAsync(s->U[s->cur], w, bytes * s->np, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->bg, w + (size_t)s->np * s->N, bytes * s->nb, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_get_grid(const smaug_t* s, double* pos, double* delx) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  if (pos) memcpy(pos, s->h_pos.data(), sizeof(double) * s->h_pos.size());
+  if (delx) memcpy(delx, s->h_delx.data(), sizeof(double) * s->h_delx.size());
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_get_clock(const smaug_t* cs, int* it, double* t, double* qt, double* dt_last) {
+  smaug_t* s = const_cast<smaug_t*>(cs);
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  double dts[2];
+  HIPCHK(hipMemcpyAsync(dts, s->dts, sizeof dts, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (it) *it = s->it;
+  if (t) *t = dts[1];
+  if (qt) *qt = s->qt;
+  if (dt_last) *dt_last = dts[0];
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_state_device_ptr(const smaug_t* s, void** dptr) {
+  if (!s || !dptr) return fail(SMAUG_E_ARG, "null argument");
+  *dptr = s->U[s->cur];
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_destroy(smaug_t* s) {
+  if (!s) return SMAUG_OK;
+  hipStreamSynchronize(s->stream);
+  free_ctx(s);
+  return SMAUG_OK;
+}
+
+extern "C" const char* smaug_last_error(void) { return g_err.c_str(); }
+extern "C" const char* smaug_version(void) { return "smaug-hip 0.1 (gfx950)"; }
+
+// Times nsteps steps with HIP events on the context stream; reports the mean
+// device time of one k_prim, one k_rhs and one full step.
+extern "C" int smaug_time_kernels(smaug_t* s, int nsteps, double* ms_prim, double* ms_rhs, double* ms_step) {
+  if (!s || nsteps < 1) return fail(SMAUG_E_ARG, "bad arguments");
+  const int nst = s->p.rkon == 1 ? 4 : 1;
+  hipEvent_t ev[2 * 4 * 2 + 2];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  double tp = 0, tr = 0, ts = 0;
+  for (int n = 0; n < nsteps; n++) {
+    s->qt = s->qt + s->p.dt;
+    const double tdep = driver_tdep(s);
+    HIPCHK(hipEventRecord(ev[16], s->stream));
+    for (int st = 0; st < nst; st++) {
+      double *win, *base, *wout;
+      const double *r1 = nullptr, *r2 = nullptr, *r3 = nullptr;
+      double hf;
+      if (s->p.rkon == 0) {
+        win = base = s->U[s->cur];
+        wout = s->U[s->cur ^ 1];
+        hf = 1.0;
+      } else {
+        win = s->U[st];
+        base = s->U[0];
+        wout = st == 3 ? s->U[0] : s->U[st + 1];
+        hf = st == 2 ? 1.0 : (st == 3 ? 1.0 / 6.0 : 0.5);
+        if (st == 3) {
+          r1 = s->U[1];
+          r2 = s->U[2];
+          r3 = s->U[3];
+        }
+      }
+      const int slot = s->cslot;
+      HIPCHK(hipEventRecord(ev[4 * st + 0], s->stream));
+      launch_prim(s->G, s->stream, win, s->bg, s->prim, s->cmax + 4 * slot, s->flag);
+      HIPCHK(hipEventRecord(ev[4 * st + 1], s->stream));
+      if (st == 0)
+        launch_dt(s->stream, s->cmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->dts);
+      RhsArgs A = rhs_args(s, win, base, wout, hf, tdep, slot);
+      A.r1 = r1;
+      A.r2 = r2;
+      A.r3 = r3;
+      HIPCHK(hipEventRecord(ev[4 * st + 2], s->stream));
+      launch_rhs(s->G, s->stream, A);
+      HIPCHK(hipEventRecord(ev[4 * st + 3], s->stream));
+      s->cslot ^= 1;
+    }
+    HIPCHK(hipEventRecord(ev[17], s->stream));
+    HIPCHK(hipEventSynchronize(ev[17]));
+    for (int st = 0; st < nst; st++) {
+      float a = 0, b = 0;
+      HIPCHK(hipEventElapsedTime(&a, ev[4 * st + 0], ev[4 * st + 1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[4 * st + 2], ev[4 * st + 3]));
+      tp += a;
+      tr += b;
+    }
+    float c = 0;
+    HIPCHK(hipEventElapsedTime(&c, ev[16], ev[17]));
+    ts += c;
+    if (s->p.rkon == 0) s->cur ^= 1;
+    s->it++;
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  if (ms_prim) *ms_prim = tp / (nsteps * nst);
+  if (ms_rhs) *ms_rhs = tr / (nsteps * nst);
+  if (ms_step) *ms_step = ts / nsteps;
+  return check_flag(s);
+}

@@ -1,0 +1,150 @@
+"""ctypes binding of the C-ABI (include/smaug_hip.h) and the host-side
+driver loop.
+
+The engine runs only through the HIP library ``libsmaughip.so``; there is no
+CPU fallback: a missing library or a failed device call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .params import RunParams, nvar
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsmaughip.so")
+_lib = None
+
+EXPORTS = [
+    "smaug_create", "smaug_step", "smaug_cfl", "smaug_fill_ghosts", "smaug_get_state",
+    "smaug_set_state", "smaug_get_grid", "smaug_get_clock", "smaug_state_device_ptr",
+    "smaug_sync", "smaug_destroy", "smaug_last_error", "smaug_version", "smaug_time_kernels",
+]
+
+
+class SmaugError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Loads libsmaughip.so (built by smaug_amd/build.py); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SmaugError(f"{LIB_PATH} is missing: build it with `python -m smaug_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.smaug_create.argtypes = [vp, dp, dp, ctypes.POINTER(vp)]
+    L.smaug_step.argtypes = [vp, ctypes.c_int]
+    L.smaug_cfl.argtypes = [vp, dp, dp]
+    L.smaug_fill_ghosts.argtypes = [vp]
+    L.smaug_get_state.argtypes = [vp, dp]
+    L.smaug_set_state.argtypes = [vp, dp]
+    L.smaug_get_grid.argtypes = [vp, dp, dp]
+    L.smaug_get_clock.argtypes = [vp, ip, dp, dp, dp]
+    L.smaug_state_device_ptr.argtypes = [vp, ctypes.POINTER(vp)]
+    L.smaug_sync.argtypes = [vp]
+    L.smaug_destroy.argtypes = [vp]
+    L.smaug_last_error.restype = ctypes.c_char_p
+    L.smaug_version.restype = ctypes.c_char_p
+    L.smaug_time_kernels.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype if name in (
+            "smaug_last_error", "smaug_version") else ctypes.c_int
+    _lib = L
+    return L
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        msg = load().smaug_last_error().decode(errors="replace")
+        raise SmaugError(f"{what} failed ({rc}): {msg}")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Engine:
+    """One simulation on the GPU: state, grid and the time loop."""
+
+    def __init__(self, rp: RunParams, w0: np.ndarray, pos: np.ndarray):
+        L = load()
+        self.rp = rp
+        self._cp = rp.to_c()
+        self.nd = rp.ndim
+        self.N = rp.ncells
+        self.nvar = nvar(rp.ndim)
+        w0 = np.ascontiguousarray(w0, dtype=np.float64).reshape(self.nvar, self.N)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nd, self.N)
+        h = ctypes.c_void_p()
+        _chk(L.smaug_create(ctypes.byref(self._cp), _dp(w0), _dp(pos), ctypes.byref(h)), "smaug_create")
+        self._h = h
+
+    def step(self, n: int = 1):
+        _chk(load().smaug_step(self._h, int(n)), "smaug_step")
+
+    def sync(self):
+        _chk(load().smaug_sync(self._h), "smaug_sync")
+
+    def state(self) -> np.ndarray:
+        w = np.empty((self.nvar, self.N))
+        _chk(load().smaug_get_state(self._h, _dp(w)), "smaug_get_state")
+        return w
+
+    def set_state(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        _chk(load().smaug_set_state(self._h, _dp(w)), "smaug_set_state")
+
+    def fill_ghosts(self):
+        _chk(load().smaug_fill_ghosts(self._h), "smaug_fill_ghosts")
+
+    def grid(self) -> Tuple[np.ndarray, np.ndarray]:
+        pos = np.empty((self.nd, self.N))
+        dx = np.empty((self.nd, self.N))
+        _chk(load().smaug_get_grid(self._h, _dp(pos), _dp(dx)), "smaug_get_grid")
+        return pos, dx
+
+    def cfl(self) -> Tuple[np.ndarray, float]:
+        cm = np.zeros(3)
+        dt = ctypes.c_double()
+        _chk(load().smaug_cfl(self._h, _dp(cm), ctypes.byref(dt)), "smaug_cfl")
+        return cm[: self.nd], dt.value
+
+    def clock(self):
+        it, t, qt, dt = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _chk(load().smaug_get_clock(self._h, ctypes.byref(it), ctypes.byref(t), ctypes.byref(qt),
+                                    ctypes.byref(dt)), "smaug_get_clock")
+        return it.value, t.value, qt.value, dt.value
+
+    def device_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        _chk(load().smaug_state_device_ptr(self._h, ctypes.byref(p)), "smaug_state_device_ptr")
+        return p.value
+
+    def time_kernels(self, nsteps: int) -> Tuple[float, float, float]:
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _chk(load().smaug_time_kernels(self._h, int(nsteps), ctypes.byref(a), ctypes.byref(b),
+                                       ctypes.byref(c)), "smaug_time_kernels")
+        return a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().smaug_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
