@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the SMAUG MHD time step on one MI355X.
+
+Workload: the 3D gravity-stratified solar atmosphere (hydrostatic tanh
+temperature profile, uniform vertical field, contcd4 boundaries,
+hyperdiffusion, vertical-velocity driver) at 256^3 interior cells (260^3
+with ghosts), four-step RK, with a seeded smoothed perturbation so every
+hyperdiffusion branch is live.  One "step" is one RK4 time step.
+
+Prints one JSON line: whole-job cell-updates/s (interior cells x steps /
+wall time, state resident in HBM), the roofline of the dominant kernel
+(k_rhs) measured with HIP events on the engine's stream, and a CPU baseline
+(the oracle, a restatement of the reference algorithm) timed on a bounded
+sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("cell-updates/s (RK4) at 256³, mixed vs FP64 on GPU; adjoint/forward time ratio")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per interior cell of one k_rhs launch (one RK stage,
+# pass B of the two-pass stage): 1664 B per cell-update of an RK4 step minus
+# the wave-speed pass (13 doubles per stage), divided over the 4 stages.
+RHS_BYTES_PER_CELL = (1664 - 4 * 13 * 8) / 4.0
+PRIM_BYTES_PER_CELL = 13 * 8.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rhs_stratified256_traffic.txt")
+
+
+def read_traffic():
+    """HBM bytes per k_rhs dispatch from the committed rocprofv3 --pmc summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE, encoding="utf-8") as fh:
+        for line in fh:
+            m = re.match(r"\s*traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+-]+)", line)
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def cpu_baseline(size: int, steps: int):
+    """Oracle (CPU restatement of the reference step) on a bounded sample."""
+    from oracle.pyoracle import Oracle, num_threads
+    from smaug_amd import configs
+
+    rp = configs.stratified_params(ni=(size, size, size), rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    orc = Oracle(rp, w, pos)
+    t0 = time.perf_counter()
+    orc.step(steps)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return {
+        "value": size ** 3 * steps / dt,
+        "unit": "cell-updates/s",
+        "cores": num_threads(),
+        "kind": "port",
+        "sample": f"oracle RK4, {size}^3 interior, {steps} step(s), same stratified workload, {dt:.1f} s",
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256, help="interior cells per dim")
+    ap.add_argument("--kernel-steps", type=int, default=3, help="steps timed per kernel with events")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-size", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build holds the whole domain on a single MI355X "
+              "(multi-GPU halo exchange is not part of it). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU (torch.cuda.is_available() is False)", file=sys.stderr)
+        return 1
+    from smaug_amd import configs
+    from smaug_amd.engine import Engine
+
+    S = args.size
+    rp = configs.stratified_params(ni=(S, S, S), rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    print(f"[bench] state built: {S}^3 interior, {rp.ncells} cells incl. ghosts", file=sys.stderr, flush=True)
+    eng = Engine(rp, w, pos)
+    del w, pos
+    eng.step(args.warmup)
+    eng.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    cells = S ** 3
+    value = cells * args.steps / wall
+    ms_step = wall / args.steps * 1e3
+    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    ms_prim, ms_rhs, ms_evstep = eng.time_kernels(args.kernel_steps)
+    eng.close()
+    rhs_bytes = RHS_BYTES_PER_CELL * cells
+    achieved = rhs_bytes / (ms_rhs * 1e-3) / 1e9
+    roof = {
+        "kernel": "k_rhs",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": read_traffic(),
+        "algorithmic_bytes_per_launch": rhs_bytes,
+        "ms_per_launch": ms_rhs,
+        "k_prim_ms_per_launch": ms_prim,
+        "k_prim_GBps": PRIM_BYTES_PER_CELL * rp.ncells / (ms_prim * 1e-3) / 1e9,
+        "event_ms_per_step": ms_evstep,
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_size, args.cpu_steps)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D gravity-stratified atmosphere {S}^3 interior ({S + 4}^3 with ghosts), "
+                        "RK4, FP64, hyperdiffusion, vertical driver, seeded perturbation",
+            "interior_cells": cells,
+            "precision": "fp64",
+            "scheme": "rk4",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -13,13 +13,14 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsmaughip.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("smaug_kernels.hip", "smaug_host.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("smaug_kernels.hip", "smaug_rhs.hip", "smaug_host.hip")]
 ARCH = os.environ.get("SMAUG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # unfused double arithmetic, as the reference's CPU build
-    "-ffp-contract=off",
+    # fused multiply-add allowed: parity with the unfused oracle is held to a
+    # tolerance (relative L2 per variable), not bitwise
+    "-ffp-contract=fast",
     # no packed FP32 instructions (they are not used on purpose anywhere)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-value", "-Wno-pass-failed",

@@ -128,12 +128,18 @@ def stratified(rp: RunParams, b_tesla: float = 0.01, perturb: Optional[int] = No
     3-point box filter: drho/rho_b ~ N(0, amp), de ~ N(0, amp) p/(gamma-1),
     v ~ N(0, amp c_s), db ~ N(0, amp |b_b|)."""
     coords = _mesh(rp)
-    z = coords[0]
+    n0 = rp.n[0]
     gam = rp.gamma
-    p, rho_b = hydrostatic(z, grav=-rp.g[0] if rp.g[0] != 0 else 274.0)
+    # the background depends on dim 0 (height) only: build it on the axis
+    zax = coords[0][:n0]
+    p_ax, rho_ax = hydrostatic(zax, grav=-rp.g[0] if rp.g[0] != 0 else 274.0)
+    reps = rp.ncells // n0
+    p = np.tile(p_ax, reps)
+    rho_b = np.tile(rho_ax, reps)
     bz = b_tesla / math.sqrt(MU0)
     nd = rp.ndim
-    w = np.zeros((nvar(nd), z.size))
+    ncell = rp.ncells
+    w = np.zeros((nvar(nd), ncell))
     eb = p / (gam - 1.0) + 0.5 * bz * bz
     i_eb, i_rhob, i_bb = 2 + 2 * nd, 3 + 2 * nd, 4 + 2 * nd
     w[i_eb] = eb
@@ -143,22 +149,28 @@ def stratified(rp: RunParams, b_tesla: float = 0.01, perturb: Optional[int] = No
         rng = np.random.default_rng(perturb)
         shape = tuple(rp.n[::-1])
 
+        taps = np.array([1.0, 3.0, 6.0, 7.0, 6.0, 3.0, 1.0]) / 27.0
+
         def smooth(a):
-            a = a.reshape(shape)
-            for _ in range(3):
-                for ax in range(a.ndim):
-                    a = (np.roll(a, 1, axis=ax) + a + np.roll(a, -1, axis=ax)) / 3.0
-            return a.ravel()
+            # three passes of the periodic 3-point box filter along every axis,
+            # applied as the equivalent 7-tap kernel [1,3,6,7,6,3,1]/27
+            a = a.reshape(shape).astype(np.float32)
+            for ax in range(a.ndim):
+                out = np.zeros_like(a)
+                for t, wgt in zip(range(-3, 4), taps):
+                    out += np.float32(wgt) * np.roll(a, t, axis=ax)
+                a = out
+            return a.ravel().astype(np.float64)
 
         cs = np.sqrt(gam * p / rho_b)
-        drho = smooth(rng.normal(0.0, amp, z.size)) * rho_b
+        drho = smooth(rng.normal(0.0, amp, ncell)) * rho_b
         w[0] = drho
         for d in range(nd):
-            v = smooth(rng.normal(0.0, amp, z.size)) * cs
+            v = smooth(rng.normal(0.0, amp, ncell)) * cs
             w[1 + d] = (rho_b + drho) * v
-        de = smooth(rng.normal(0.0, amp, z.size)) * p / (gam - 1.0)
+        de = smooth(rng.normal(0.0, amp, ncell)) * p / (gam - 1.0)
         w[1 + nd] = de
         for d in range(nd):
-            w[2 + nd + d] = smooth(rng.normal(0.0, amp, z.size)) * bz
+            w[2 + nd + d] = smooth(rng.normal(0.0, amp, ncell)) * bz
     pos = np.stack(coords)
     return w, pos

@@ -33,6 +33,7 @@ struct Geo {
   int hyper;             // PHYPERDIFMOM
   int driver;
   double drv[8];
+  int kver;              // k_rhs variant: 2 tiled (default), 1 per-cell
 };
 
 // evolved-field indices
@@ -91,6 +92,18 @@ __host__ __device__ inline int bc_images(int type, int y, int n, int* out) {
   return m;
 }
 
+// zeroing of grad3dn at interior cells: first/last two interior layers
+__device__ inline bool z4(const Geo& G, int idx, int d, int x) {
+  const int n = G.n[d];
+  return ((G.zm[idx] >> d) & 1u) && (x == 2 || x == 3 || x == n - 4 || x == n - 3);
+}
+// zeroing of grad1l3n/grad1r3n/grad13n at cells whose other coordinates
+// are inside 1..n-2: the first and last interior layer
+__device__ inline bool z1(const Geo& G, int idx, int d, int x) {
+  return ((G.zm[idx] >> d) & 1u) && (x == 2 || x == G.n[d] - 3);
+}
+
+
 // ---------------------------------------------------------------------------
 // Right-chain hyperdiffusion halo (quirk Q7).  The reference writes q of
 // cell t into the padded (n+2)^d array at linear index t + C (unpadded
@@ -124,6 +137,7 @@ struct RhsArgs {
   const double* bg;     // background
   const double* prim;   // k_prim output of win
   const double* delx;   // nd*N
+  const double* rdelx;  // nd*N, 1/delx
   const double* pos;    // nd*N
   const double* base;   // stage base (w^n), read at the cell only
   const double* r1;     // RK combination inputs (final stage only, else null)
@@ -143,6 +157,7 @@ void launch_zero3(hipStream_t st, unsigned long long* p);
 void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,
                double dtcap, int moddton, double* dts);
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
+void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A);
 void launch_fill(const Geo& G, hipStream_t st, double* w);
 
 }  // namespace smaug

@@ -43,6 +43,7 @@ struct smaug_ctx {
   double* bg = nullptr;
   double* prim = nullptr;
   double* delx = nullptr;
+  double* rdelx = nullptr;
   double* pos = nullptr;
   unsigned long long* cmax = nullptr;  // 3 slots of 4
   double* dts = nullptr;               // dt, t
@@ -174,6 +175,8 @@ static void make_geo(const smaug_params* p, Geo& G) {
   G.hyper = p->hyperdifmom;
   G.driver = p->driver;
   for (int i = 0; i < 8; i++) G.drv[i] = p->drv[i];
+  const char* kv = getenv("SMAUG_RHS");
+  G.kver = (kv && kv[0] == '1') ? 1 : 2;
 }
 
 static void free_ctx(smaug_t* s) {
@@ -183,6 +186,7 @@ static void free_ctx(smaug_t* s) {
   hipFree(s->bg);
   hipFree(s->prim);
   hipFree(s->delx);
+  hipFree(s->rdelx);
   hipFree(s->pos);
   hipFree(s->cmax);
   hipFree(s->dts);
@@ -218,6 +222,7 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   ok &= hipMalloc(&s->bg, bytes * s->nb) == hipSuccess;
   ok &= hipMalloc(&s->prim, bytes * s->npr) == hipSuccess;
   ok &= hipMalloc(&s->delx, bytes * s->nd) == hipSuccess;
+  ok &= hipMalloc(&s->rdelx, bytes * s->nd) == hipSuccess;
   ok &= hipMalloc(&s->pos, bytes * s->nd) == hipSuccess;
   ok &= hipMalloc(&s->cmax, sizeof(unsigned long long) * 12) == hipSuccess;
   ok &= hipMalloc(&s->dts, sizeof(double) * 2) == hipSuccess;
@@ -227,12 +232,15 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
     return fail(SMAUG_E_NOMEM, "device allocation failed");
   }
   initgrid_host(s->nd, p->n, pos, s->h_pos, s->h_delx);
+  std::vector<double> h_rdelx(s->h_delx.size());
+  for (size_t q = 0; q < h_rdelx.size(); q++) h_rdelx[q] = 1.0 / s->h_delx[q];
   const double dts0[2] = {p->moddton == 1 ? 1.0e-8 : p->dt, 0.0};  // src/iosmaug.c:105,333-336
   const unsigned long long z12[12] = {0};
   const int z = 0;
   if (hipMemcpy(s->U[0], w0, bytes * s->np, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->bg, w0 + (size_t)s->np * s->N, bytes * s->nb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->delx, s->h_delx.data(), bytes * s->nd, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->rdelx, h_rdelx.data(), bytes * s->nd, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->pos, s->h_pos.data(), bytes * s->nd, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->cmax, z12, sizeof z12, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->dts, dts0, sizeof dts0, hipMemcpyHostToDevice) != hipSuccess ||
@@ -260,6 +268,7 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
   A.bg = s->bg;
   A.prim = s->prim;
   A.delx = s->delx;
+  A.rdelx = s->rdelx;
   A.pos = s->pos;
   A.base = base;
   A.r1 = A.r2 = A.r3 = nullptr;

@@ -33,10 +33,10 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
                                               unsigned long long* __restrict__ cmax_bits,
                                               int* __restrict__ flag) {
   const long long N = G.N;
-  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
   double mx[3] = {0.0, 0.0, 0.0};
   int bad = 0;
-  if (c < N) {
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += stride) {
     const double gm = G.gamma;
     const double rho = w[c], rhob = bg[(long long)bRHOB() * N + c];
     double m[ND], b[ND], bb[ND];
@@ -138,17 +138,6 @@ __global__ void k_dt(const unsigned long long* cmax_bits, int nd, double dx0, do
 // src/gradops.cu:437-602); the caller applies the Q1 zeroing.
 __device__ inline double d4(const double* __restrict__ a, long long c, long long s, double dx) {
   return ((8 * a[c + s] - 8 * a[c - s] + a[c - 2 * s] - a[c + 2 * s]) / 6.0) / (2.0 * dx);
-}
-
-// zeroing of grad3dn at interior cells: first/last two interior layers
-__device__ inline bool z4(const Geo& G, int idx, int d, int x) {
-  const int n = G.n[d];
-  return ((G.zm[idx] >> d) & 1u) && (x == 2 || x == 3 || x == n - 4 || x == n - 3);
-}
-// zeroing of grad1l3n/grad1r3n/grad13n at cells whose other coordinates
-// are inside 1..n-2: the first and last interior layer
-__device__ inline bool z1(const Geo& G, int idx, int d, int x) {
-  return ((G.zm[idx] >> d) & 1u) && (x == 2 || x == G.n[d] - 3);
 }
 
 template <int ND>
@@ -557,7 +546,9 @@ __global__ __launch_bounds__(256) void k_fill(Geo G, double* __restrict__ w) {
 
 void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg, double* prim,
                  unsigned long long* cmax, int* flag) {
-  const unsigned nb = (unsigned)((G.N + 255) / 256);
+  // grid-stride: at most 2048 blocks so the per-block atomics stay few
+  long long nb0 = (G.N + 255) / 256;
+  const unsigned nb = (unsigned)(nb0 < 2048 ? nb0 : 2048);
   if (G.nd == 3)
     hipLaunchKernelGGL(k_prim<3>, dim3(nb), dim3(256), 0, st, G, w, bg, prim, cmax, flag);
   else
@@ -574,13 +565,20 @@ void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const dou
                      dtcap, moddton, dts);
 }
 
-void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A) {
+void launch_rhs_v1(const Geo& G, hipStream_t st, const RhsArgs& A) {
   const int ni0 = G.n[0] - 4, ni1 = G.n[1] - 4;
   dim3 grid((ni0 + 63) / 64, (ni1 + 3) / 4, G.nd == 3 ? G.n[2] - 4 : 1);
   if (G.nd == 3)
     hipLaunchKernelGGL(k_rhs<3>, grid, dim3(64, 4), 0, st, G, A);
   else
     hipLaunchKernelGGL(k_rhs<2>, grid, dim3(64, 4), 0, st, G, A);
+}
+
+void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A) {
+  if (G.kver == 1)
+    launch_rhs_v1(G, st, A);
+  else
+    launch_rhs_tiled(G, st, A);
 }
 
 void launch_fill(const Geo& G, hipStream_t st, double* w) {

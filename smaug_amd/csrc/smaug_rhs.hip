@@ -1,0 +1,583 @@
+// smaug_rhs.hip -- tiled right-hand side of one SMAUG stage (k_rhs).
+//
+// One workgroup owns a tile of interior cells (8x8x4 in 3D, 16x16 in 2D,
+// 256 threads, one cell each) and builds every term of the reference step
+// for it from LDS-staged data, so each intermediate is computed once:
+//   1. flux phase, per direction d: the eight fluxes F_f^d of the tile
+//      extended by two cells along d go to LDS, then each thread takes the
+//      4th-order divergence (centdiff1/centdiff2).
+//   2. hyperdiffusion phase, per variable q (rho, e_int, v_f, b_f): q is
+//      staged as the tile plus a 3-cell halo (with the reference's halo
+//      substitutions of quirks Q7/Q8); for every direction the coefficients
+//      nu_{i+-1/2} and the cross-term products of the off-diagonal momentum
+//      and magnetic sources are formed once per cell of the tile extended by
+//      one cell along the cross direction, then each thread applies the
+//      diagonal and off-diagonal sources.  The v_f phase also applies the
+//      background energy terms, which need gradients of v_f only.
+//   3. gravity, the stage update, the user source, the RK combination and
+//      the ghost-cell images (boundaries fused).
+#include <math.h>
+
+#include "../../include/smaug_hip.h"
+#include "smaug_dev.h"
+
+namespace smaug {
+
+template <int ND>
+struct Tile;
+template <>
+struct Tile<3> {
+  static constexpr int X = 8, Y = 8, Z = 4;
+};
+template <>
+struct Tile<2> {
+  static constexpr int X = 16, Y = 16, Z = 1;
+};
+
+constexpr int imax(int a, int b) { return a > b ? a : b; }
+
+template <int ND>
+struct Lay {
+  static constexpr int TX = Tile<ND>::X, TY = Tile<ND>::Y, TZ = Tile<ND>::Z;
+  static constexpr int NP = 2 + 2 * ND;
+  // q box: tile + 3-cell halo
+  static constexpr int QX = TX + 6, QY = TY + 6, QZ = ND == 3 ? TZ + 6 : 1;
+  static constexpr int QN = QX * QY * QZ;
+  // aux box: tile + 1-cell halo (rho_t, m_0..m_{nd-1})
+  static constexpr int AX = TX + 2, AY = TY + 2, AZ = ND == 3 ? TZ + 2 : 1;
+  static constexpr int AN = AX * AY * AZ;
+  // region: tile extended by one cell along one axis
+  static constexpr int RN = ND == 3 ? imax(imax((TX + 2) * TY * TZ, TX * (TY + 2) * TZ), TX * TY * (TZ + 2))
+                                    : imax((TX + 2) * TY, TX * (TY + 2));
+  // flux region: tile extended by two cells along one axis
+  static constexpr int FN = ND == 3 ? imax(imax((TX + 4) * TY * TZ, TX * (TY + 4) * TZ), TX * TY * (TZ + 4))
+                                    : imax((TX + 4) * TY, TX * (TY + 4));
+  static constexpr int HYP = QN + (1 + ND) * AN + 4 * RN;
+  static constexpr int FLX = NP * FN;
+  static constexpr int SMEM = HYP > FLX ? HYP : FLX;
+};
+
+__device__ inline long long gix(const Geo& G, int a, int b, int c) {
+  return ((long long)c * G.n[1] + b) * G.n[0] + a;
+}
+
+template <int ND>
+__device__ inline bool in_grid(const Geo& G, const int* x) {
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if (x[d] < 0 || x[d] >= G.n[d]) return false;
+  return true;
+}
+
+template <int ND>
+__device__ inline bool interior_c(const Geo& G, const int* x) {
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if (x[d] < 2 || x[d] > G.n[d] - 3) return false;
+  return true;
+}
+
+// q at grid coordinates x, one step past the grid allowed in one direction:
+// coordinate -1 reads coordinate 5 (left-chain halo, quirk Q8), coordinate n
+// reads the right-chain stray halo (quirk Q7).
+template <int ND>
+__device__ inline double qval(const Geo& G, const double* __restrict__ qa, const int* x) {
+  int out = -1, nout = 0;
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if (x[d] < 0 || x[d] >= G.n[d]) {
+      out = d;
+      nout++;
+    }
+  if (nout == 0) return qa[gix(G, x[0], x[1], ND == 3 ? x[2] : 0)];
+  if (nout > 1) return 0.0;
+  int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
+  if (x[out] == -1) {
+    y[out] = 5;
+    return qa[gix(G, y[0], y[1], y[2])];
+  }
+  if (x[out] == G.n[out]) {
+    int pc[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) pc[q] = y[q] + 1;
+    pc[out] = G.n[out] + 1;
+    const long long h = halo_r_src(G, pc);
+    return h < 0 ? 0.0 : qa[h];
+  }
+  return 0.0;
+}
+
+// nu_{i+1/2}, nu_{i-1/2} from a 7-cell window of q in LDS (centre index ci,
+// stride s): third differences at 4 interfaces, first differences at 6,
+// maxima over 3 and 5 interfaces (src/hyperdifvisc1r.cu:571-565 chain).
+__device__ inline void nu_win(const double* __restrict__ qb, int ci, int s, double dxk, double& nur,
+                              double& nul) {
+  double Q[7];
+#pragma unroll
+  for (int o = 0; o < 7; o++) Q[o] = qb[ci + (o - 3) * s];
+  double D3[5], D1[6];
+#pragma unroll
+  for (int a = 1; a <= 4; a++) D3[a] = fabs(3.0 * (Q[a + 1] - Q[a]) - (Q[a + 2] - Q[a - 1]));
+#pragma unroll
+  for (int a = 0; a <= 5; a++) D1[a] = fabs(Q[a + 1] - Q[a]);
+  // maxima over the right (i+1/2) and left (i-1/2) windows share their
+  // middle; fmax ignores NaN as the reference's "if (v > max)" does
+  const double t3 = fmax(D3[2], D3[3]);
+  const double t5 = fmax(fmax(D1[1], D1[2]), fmax(D1[3], D1[4]));
+  const double m3r = fmax(fmax(t3, D3[4]), 0.0), m3l = fmax(fmax(t3, D3[1]), 0.0);
+  const double m5r = fmax(fmax(t5, D1[5]), 0.0), m5l = fmax(fmax(t5, D1[0]), 0.0);
+  nur = m5r > 0 ? dxk * m3r / m5r : 0.0;
+  nul = m5l > 0 ? dxk * m3l / m5l : 0.0;
+}
+
+template <int ND>
+__device__ inline void stage_q(const Geo& G, const double* __restrict__ qa, double* __restrict__ qb,
+                               const int* o) {
+  using L = Lay<ND>;
+  // fast path: the halo box lies inside the grid, no substitutions needed;
+  // each thread owns one (a,b) column of the box and walks its planes
+  bool inside = o[0] - 3 >= 0 && o[1] - 3 >= 0 && o[0] + L::TX + 3 <= G.n[0] && o[1] + L::TY + 3 <= G.n[1];
+  if (ND == 3) inside = inside && o[2] - 3 >= 0 && o[2] + L::TZ + 3 <= G.n[2];
+  if (inside) {
+    constexpr int PL = L::QX * L::QY;
+    const long long zs = (long long)G.n[0] * G.n[1];
+    for (int l = threadIdx.x; l < PL; l += 256) {
+      const int a = l % L::QX, b = l / L::QX;
+      const double* src = qa + gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
+#pragma unroll
+      for (int cz = 0; cz < L::QZ; cz++) qb[cz * PL + l] = src[cz * zs];
+    }
+    return;
+  }
+  for (int l = threadIdx.x; l < L::QN; l += 256) {
+    const int a = l % L::QX, b = (l / L::QX) % L::QY, c = l / (L::QX * L::QY);
+    int x[3] = {o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 + c : 0};
+    qb[l] = qval<ND>(G, qa, x);
+  }
+}
+
+// fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,
+// src/centdiff2.cu:158-356), from v = m/rho_t and p_t of k_prim
+template <int ND>
+__device__ inline void fluxes2(const Geo& G, const double* __restrict__ w, const double* __restrict__ bg,
+                               const double* __restrict__ pr, long long cc, int d, double* F) {
+  const long long N = G.N;
+  double v[ND], b[ND], bb[ND];
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    v[q] = pr[(long long)pV(q) * N + cc];
+    b[q] = w[(long long)fB(ND, q) * N + cc];
+    bb[q] = bg[(long long)bBB(q) * N + cc];
+  }
+  double vd = v[0], bd = b[0], bbd = bb[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++)
+    if (q == d) {
+      vd = v[q];
+      bd = b[q];
+      bbd = bb[q];
+    }
+  const double rho = w[cc], rhob = bg[(long long)bRHOB() * N + cc];
+  const double pt = pr[(long long)pPT(ND) * N + cc];
+  F[fRHO()] = vd * rho + rhob * vd;
+#pragma unroll
+  for (int f = 0; f < ND; f++) {
+    const double mf = w[(long long)fMOM(f) * N + cc];
+    const double mag = -(b[f] * bbd + bb[f] * bd) - b[f] * bd;
+    F[fMOM(f)] = vd * mf + (f == d ? pt + mag : mag);
+  }
+  double bbv = bb[0] * v[0], bv = b[0] * v[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++) {
+    bbv = bbv + bb[q] * v[q];
+    bv = bv + b[q] * v[q];
+  }
+  const double e = w[(long long)fE(ND) * N + cc], eb = bg[(long long)bEB() * N + cc];
+  F[fE(ND)] = vd * e + ((vd * pt - bd * bbv - bbd * bv) + vd * eb - bd * bv);
+#pragma unroll
+  for (int f = 0; f < ND; f++)
+    F[fB(ND, f)] = (f == d) ? 0.0 : vd * b[f] - (bd + bbd) * v[f] + bb[f] * vd;
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
+  using L = Lay<ND>;
+  constexpr int NP = L::NP;
+  constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
+  __shared__ double sm[L::SMEM];
+  const int tid = threadIdx.x;
+  if (A.zero_next && blockIdx.x == 0 && tid < 3) A.zero_next[tid] = 0ull;
+  // XCD-aware tile order: the hardware deals consecutive workgroups to the
+  // 8 XCDs in turn, so workgroup b runs on XCD b%8; give XCD x the
+  // contiguous block of tiles [x*nb/8, (x+1)*nb/8) so neighbouring tiles
+  // share that XCD's L2 for their halos.
+  const int gx = (G.n[0] - 4 + TX - 1) / TX, gy = (G.n[1] - 4 + TY - 1) / TY;
+  const int gz = ND == 3 ? (G.n[2] - 4 + TZ - 1) / TZ : 1;
+  const int nb = gx * gy * gz;
+  const int b = blockIdx.x;
+  int t = b;
+  if ((nb & 7) == 0) t = (b & 7) * (nb >> 3) + (b >> 3);
+  const int tbx = t % gx, tby = (t / gx) % gy, tbz = t / (gx * gy);
+  const int o[3] = {2 + tbx * TX, 2 + tby * TY, ND == 3 ? 2 + tbz * TZ : 0};
+  const int lc[3] = {tid % TX, (tid / TX) % TY, ND == 3 ? tid / (TX * TY) : 0};
+  int x[3] = {o[0] + lc[0], o[1] + lc[1], o[2] + lc[2]};
+  const bool valid = interior_c<ND>(G, x);
+  const long long N = G.N;
+  const long long c = gix(G, x[0], x[1], x[2]);
+  const double* __restrict__ w = A.win;
+  const double* __restrict__ bg = A.bg;
+  const double* __restrict__ pr = A.prim;
+  const unsigned T = G.terms;
+  double acc[NP];
+#pragma unroll
+  for (int f = 0; f < NP; f++) acc[f] = 0.0;
+  double dx[ND], rdx[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    dx[d] = valid ? A.delx[(long long)d * N + c] : 1.0;
+    rdx[d] = valid ? A.rdelx[(long long)d * N + c] : 1.0;
+  }
+
+  // ------------------------------------------------------------ fluxes
+  if (T & SMAUG_T_FLUX) {
+    double* FB = sm;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const int ex[3] = {d == 0 ? 2 : 0, d == 1 ? 2 : 0, d == 2 ? 2 : 0};
+      const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
+      for (int l = tid; l < rx * ry * rz; l += 256) {
+        const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
+        int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
+        double F[NP];
+        if (in_grid<ND>(G, xr)) {
+          fluxes2<ND>(G, w, bg, pr, gix(G, xr[0], xr[1], xr[2]), d, F);
+        } else {
+#pragma unroll
+          for (int f = 0; f < NP; f++) F[f] = 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < NP; f++) FB[f * L::FN + l] = F[f];
+      }
+      __syncthreads();
+      if (valid && !z4(G, zFLUX(ND), d, x[d])) {
+        const int s = d == 0 ? 1 : (d == 1 ? rx : rx * ry);
+        const int li = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+        const double inv = rdx[d] * (1.0 / 12.0);
+#pragma unroll
+        for (int f = 0; f < NP; f++) {
+          const double* Fp = FB + f * L::FN + li;
+          acc[f] -= ((8 * Fp[s] - 8 * Fp[-s]) + (Fp[-2 * s] - Fp[2 * s])) * inv;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  const double rho_c = valid ? w[c] : 0.0;
+  const double rhob_c = valid ? bg[(long long)bRHOB() * N + c] : 1.0;
+  const double rt_c = rho_c + rhob_c;
+
+  // ------------------------------------------------------------ hyperdiffusion + energy terms
+  double* qb = sm;
+  double* ax = sm + L::QN;            // (1+ND) * AN: rho_t, m_0..
+  double* NR = ax + (1 + ND) * L::AN; // region arrays
+  double* NL = NR + L::RN;
+  double* GA = NL + L::RN;
+  double* GB = GA + L::RN;
+  // tile-cell indices in the q box and the aux box
+  const int qc = (lc[0] + 3) + L::QX * ((lc[1] + 3) + L::QY * (ND == 3 ? lc[2] + 3 : 0));
+  const int qs[3] = {1, L::QX, L::QX * L::QY};
+  const int ac = (lc[0] + 1) + L::AX * ((lc[1] + 1) + L::AY * (ND == 3 ? lc[2] + 1 : 0));
+  const int as[3] = {1, L::AX, L::AX * L::AY};
+
+  const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
+  const bool want_eterm = (T & SMAUG_T_ETERM) != 0;
+  double cm[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) cm[d] = __longlong_as_double((long long)A.cmax_bits[d]);
+
+  if (want_mom) {  // rho_t and momenta with a 1-cell halo
+    for (int l = tid; l < L::AN; l += 256) {
+      const int a = l % L::AX, b = (l / L::AX) % L::AY, cz = l / (L::AX * L::AY);
+      int xa[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
+      if (in_grid<ND>(G, xa)) {
+        const long long g = gix(G, xa[0], xa[1], xa[2]);
+        ax[l] = w[g] + bg[(long long)bRHOB() * N + g];
+#pragma unroll
+        for (int q = 0; q < ND; q++) ax[(1 + q) * L::AN + l] = w[(long long)fMOM(q) * N + g];
+      } else {
+        ax[l] = 1.0;
+#pragma unroll
+        for (int q = 0; q < ND; q++) ax[(1 + q) * L::AN + l] = 0.0;
+      }
+    }
+  }
+
+  if (G.hyper && (T & SMAUG_T_HRHO)) {  // src/hyperdifrhosource1.cu
+    stage_q<ND>(G, w, qb, o);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int dim = 0; dim < ND; dim++) {
+        double nur, nul;
+        nu_win(qb, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fRHO()], nur, nul);
+        const bool z = z1(G, zRHO, dim, x[dim]);
+        const double q0 = qb[qc];
+        const double gr = z ? 0.0 : (qb[qc + qs[dim]] - q0) * rdx[dim];
+        const double gl = z ? 0.0 : (q0 - qb[qc - qs[dim]]) * rdx[dim];
+        acc[fRHO()] += (nur * gr - nul * gl) * rdx[dim];
+      }
+    }
+    __syncthreads();
+  }
+  if (G.hyper && (T & SMAUG_T_HE)) {  // src/hyperdifesource1.cu
+    stage_q<ND>(G, pr + (long long)pQE(ND) * N, qb, o);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int dim = 0; dim < ND; dim++) {
+        double nur, nul;
+        nu_win(qb, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fE(ND)], nur, nul);
+        const bool z = z1(G, zTMP1, dim, x[dim]);
+        const double q0 = qb[qc];
+        const double gr = z ? 0.0 : (qb[qc + qs[dim]] - q0) * rdx[dim];
+        const double gl = z ? 0.0 : (q0 - qb[qc - qs[dim]]) * rdx[dim];
+        acc[fE(ND)] += (nur * gr - nul * gl) * rdx[dim];
+      }
+    }
+    __syncthreads();
+  }
+
+  double ptb = 0.0, bbc[ND];
+  if (want_eterm && valid) {
+    double bb2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      bbc[d] = bg[(long long)bBB(d) * N + c];
+      bb2 = d == 0 ? bbc[d] * bbc[d] : bb2 + bbc[d] * bbc[d];
+    }
+    ptb = (G.gamma - 1) * bg[(long long)bEB() * N + c] - 0.5 * (G.gamma - 2) * bb2;
+  }
+
+  // momentum phases (q = v_f): energy terms, diagonal and off-diagonal sources
+  if (want_mom || want_eterm) {
+#pragma unroll
+    for (int f = 0; f < ND; f++) {
+      stage_q<ND>(G, pr + (long long)pV(f) * N, qb, o);
+      __syncthreads();
+      if (want_eterm && valid) {  // addenergyterms_cd2 for dir f
+        double gf[ND];
+#pragma unroll
+        for (int i2 = 0; i2 < ND; i2++) {
+          const int s = qs[i2];
+          gf[i2] = z4(G, zVEL(f), i2, x[i2])
+                       ? 0.0
+                       : ((8 * qb[qc + s] - 8 * qb[qc - s]) + (qb[qc - 2 * s] - qb[qc + 2 * s])) * (rdx[i2] * (1.0 / 12.0));
+        }
+        double t = -ptb * gf[f];
+#pragma unroll
+        for (int i2 = 0; i2 < ND; i2++) t += bbc[i2] * bbc[f] * gf[i2];
+        acc[fE(ND)] += t;
+      }
+      if (want_mom) {
+#pragma unroll
+        for (int dim = 0; dim < ND; dim++) {
+          const double K = cm[dim] * G.chyp[fMOM(f)];
+          const bool ne = f != dim;
+          double nur = 0, nul = 0;
+          const int ex[3] = {ne && f == 0 ? 1 : 0, ne && f == 1 ? 1 : 0, ne && f == 2 ? 1 : 0};
+          const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
+          const int rs[3] = {1, rx, rx * ry};
+          const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+          if (ne) {
+            for (int l = tid; l < rx * ry * rz; l += 256) {
+              const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
+              int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
+              double nr = 0, nl = 0, g7 = 0, g8 = 0;
+              if (interior_c<ND>(G, xr)) {
+                const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
+                const int ai = (a - ex[0] + 1) + L::AX * ((b - ex[1] + 1) + L::AY * (ND == 3 ? cz - ex[2] + 1 : 0));
+                const long long gq = gix(G, xr[0], xr[1], xr[2]);
+                const double dxr = A.delx[(long long)dim * N + gq], rdxr = A.rdelx[(long long)dim * N + gq];
+                nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
+                const double t5 =
+                    z1(G, zTMP4, dim, xr[dim]) ? 0.0 : (qb[qi + qs[dim]] - qb[qi - qs[dim]]) * rdxr * 0.5;
+                const double t6 = t5 * (nr + nl) * 0.25;
+                g7 = ax[ai] * t6;
+                g8 = t6 * ax[(1 + dim) * L::AN + ai];
+              }
+              NR[l] = nr;
+              NL[l] = nl;
+              GA[l] = g7;
+              GB[l] = g8;
+            }
+            __syncthreads();
+            if (valid) {
+              nur = NR[ri];
+              nul = NL[ri];
+            }
+          } else if (valid) {
+            nu_win(qb, qc, qs[dim], dx[dim] * K, nur, nul);
+          }
+          if (valid) {
+            const int s = qs[dim], sa = as[dim];
+            const bool z = z1(G, zTMP4, dim, x[dim]);
+            const double q0 = qb[qc];
+            const double gr = z ? 0.0 : (qb[qc + s] - q0) * rdx[dim];
+            const double gl = z ? 0.0 : (q0 - qb[qc - s]) * rdx[dim];
+            const double rr = (ax[ac] + ax[ac + sa]) * 0.5, rl = (ax[ac] + ax[ac - sa]) * 0.5;
+            const double* mf = ax + (1 + f) * L::AN;
+            const double mr = (mf[ac] + mf[ac + sa]) * 0.5, ml = (mf[ac] + mf[ac - sa]) * 0.5;
+            const double dmv = (rr * nur * gr - rl * nul * gl) * (rdx[dim] * 0.5);
+            const double dev = (mr * nur * gr - ml * nul * gl) * (rdx[dim] * 0.5);
+            acc[fMOM(f)] += dmv;
+            acc[fE(ND)] += dev;
+            if (!ne) {
+              acc[fMOM(f)] += dmv;
+              acc[fE(ND)] += dev;
+            } else {
+              const int sf = rs[f];
+              const double o7 = z1(G, zTMP7, f, x[f]) ? 0.0 : (GA[ri + sf] - GA[ri - sf]) * (rdx[f] * 0.5);
+              const double o8 = z1(G, zTMP8, f, x[f]) ? 0.0 : (GB[ri + sf] - GB[ri - sf]) * (rdx[f] * 0.5);
+              acc[fMOM(dim)] += o7;
+              acc[fE(ND)] += o8;
+            }
+          }
+          if (ne) __syncthreads();
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // magnetic phases (q = b_f): off-diagonal (sb=-1) and diagonal (sb=+1)
+  if (G.hyper && (T & SMAUG_T_HB)) {
+#pragma unroll
+    for (int f = 0; f < ND; f++) {
+      stage_q<ND>(G, w + (long long)fB(ND, f) * N, qb, o);
+      __syncthreads();
+#pragma unroll
+      for (int dim = 0; dim < ND; dim++) {
+        if (dim == f) continue;
+        const double K = cm[dim] * G.chyp[fB(ND, f)];
+        const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
+        const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
+        const int rs[3] = {1, rx, rx * ry};
+        const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+        const double* bd = w + (long long)fB(ND, dim) * N;
+        for (int l = tid; l < rx * ry * rz; l += 256) {
+          const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
+          int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
+          double nr = 0, nl = 0, g3 = 0, g5 = 0;
+          if (interior_c<ND>(G, xr)) {
+            const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
+            const long long gr_ = gix(G, xr[0], xr[1], xr[2]);
+            const double dxr = A.delx[(long long)dim * N + gr_], rdxr = A.rdelx[(long long)dim * N + gr_];
+            nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
+            const double t2 = z1(G, zTMP1, dim, xr[dim]) ? 0.0 : (qb[qi + qs[dim]] - qb[qi - qs[dim]]) * rdxr * 0.5;
+            const double t3 = t2 * (nl + nr) * 0.5;
+            g3 = t3;
+            g5 = t3 * bd[gr_];
+          }
+          NR[l] = nr;
+          NL[l] = nl;
+          GA[l] = g3;
+          GB[l] = g5;
+        }
+        __syncthreads();
+        if (valid) {
+          const double nur = NR[ri], nul = NL[ri];
+          const int sf = rs[f];
+          const double o3 = z1(G, zTMP3, f, x[f]) ? 0.0 : (GA[ri + sf] - GA[ri - sf]) * (rdx[f] * 0.5);
+          const double o5 = z1(G, zTMP5, f, x[f]) ? 0.0 : (GB[ri + sf] - GB[ri - sf]) * (rdx[f] * 0.5);
+          acc[fB(ND, dim)] += -1.0 * o3;
+          acc[fE(ND)] += -1.0 * o5;
+          const int s = qs[dim];
+          const bool z = z1(G, zTMP1, dim, x[dim]);
+          const double q0 = qb[qc];
+          const double gr = z ? 0.0 : (qb[qc + s] - q0) * rdx[dim];
+          const double gl = z ? 0.0 : (q0 - qb[qc - s]) * rdx[dim];
+          const double br = (q0 + qb[qc + s]) * 0.5, bl = (q0 + qb[qc - s]) * 0.5;
+          acc[fB(ND, f)] += (gr * nur - gl * nul) * rdx[dim];
+          acc[fE(ND)] += (br * gr * nur - bl * gl * nul) * rdx[dim];
+        }
+        __syncthreads();
+      }
+    }
+  }
+
+  if (!valid) return;
+  // ------------------------------------------------------------ gravity
+  if (T & SMAUG_T_GRAV) {
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      acc[fMOM(d)] += G.g[d] * rho_c;
+      acc[fE(ND)] += rho_c * (G.g[d] * w[(long long)fMOM(d) * N + c]) / rt_c;
+    }
+  }
+  // ------------------------------------------------------------ update, source, RK, ghosts
+  const double h = A.hfrac * A.dts[0];
+  double out[NP];
+#pragma unroll
+  for (int f = 0; f < NP; f++) out[f] = A.base[(long long)f * N + c] + h * acc[f];
+  if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_VERTICAL) {
+    const double* dv = G.drv;
+    const double zp = A.pos[c] - dv[2];
+    const double xp = A.pos[N + c] - dv[3];
+    const double ez = exp(-zp * zp / (dv[5] * dv[5]));
+    const double exx = exp(-xp * xp / (dv[6] * dv[6]));
+    double exyz = exx * ez;
+    if (ND == 3) {
+      const double yp = A.pos[2 * N + c] - dv[4];
+      const double ey = exp(-yp * yp / (dv[6] * dv[6]));
+      exyz = exx * ey * ez;
+    }
+    const double vz = dv[0] * exyz * A.tdep;
+    const double rto = out[fRHO()] + rhob_c;
+    out[fMOM(0)] += h * vz * rto;
+    out[fE(ND)] += h * (vz * vz) * rto / 2.0;
+  }
+  if (A.r1) {
+#pragma unroll
+    for (int f = 0; f < NP; f++) {
+      const long long q = (long long)f * N + c;
+      out[f] = out[f] + (A.r1[q] + 2.0 * A.r2[q] + A.r3[q] - 4.0 * A.base[q]) / 3;
+    }
+  }
+  double* __restrict__ wo = A.wout;
+#pragma unroll
+  for (int f = 0; f < NP; f++) {
+    if (f > 0 && G.bc_uniform) break;
+    int li[3][3], ln[3] = {1, 1, 1};
+#pragma unroll
+    for (int d = 0; d < 3; d++) li[d][0] = x[d];
+#pragma unroll
+    for (int d = 0; d < ND; d++) ln[d] = 1 + bc_images(G.bct[f][d], x[d], G.n[d], &li[d][1]);
+    for (int a = 0; a < ln[0]; a++)
+      for (int b2 = 0; b2 < ln[1]; b2++)
+        for (int c2 = 0; c2 < ln[2]; c2++) {
+          const long long cc = gix(G, li[0][a], li[1][b2], li[2][c2]);
+          if (G.bc_uniform) {
+#pragma unroll
+            for (int q = 0; q < NP; q++) wo[(long long)q * N + cc] = out[q];
+          } else {
+            wo[(long long)f * N + cc] = out[f];
+          }
+        }
+  }
+}
+
+void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A) {
+  if (G.nd == 3) {
+    using L = Lay<3>;
+    const int nb = ((G.n[0] - 4 + L::TX - 1) / L::TX) * ((G.n[1] - 4 + L::TY - 1) / L::TY) *
+                   ((G.n[2] - 4 + L::TZ - 1) / L::TZ);
+    hipLaunchKernelGGL(k_rhs_tiled<3>, dim3(nb), dim3(256), 0, st, G, A);
+  } else {
+    using L = Lay<2>;
+    const int nb = ((G.n[0] - 4 + L::TX - 1) / L::TX) * ((G.n[1] - 4 + L::TY - 1) / L::TY);
+    hipLaunchKernelGGL(k_rhs_tiled<2>, dim3(nb), dim3(256), 0, st, G, A);
+  }
+}
+
+}  // namespace smaug
