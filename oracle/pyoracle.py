@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+_LIB = os.environ.get("SMAUG_ORACLE_LIB", os.path.join(_HERE, "liboracle.so"))
 _lib = None
 
 
@@ -38,6 +38,8 @@ def lib():
         L.orc_get_clock.argtypes = [vp, ip, dp, dp, dp]
         L.orc_destroy.argtypes = [vp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_probe_grad.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_probe_visc.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -96,6 +98,26 @@ class Oracle:
         it, t, qt, dt = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         lib().orc_get_clock(self._h, ctypes.byref(it), ctypes.byref(t), ctypes.byref(qt), ctypes.byref(dt))
         return it.value, t.value, qt.value, dt.value
+
+    GRAD_KINDS = {"grad3dn": 0, "grad1l3n": 1, "grad1r3n": 2, "grad13n": 3}
+
+    def grad(self, kind: str, a: np.ndarray, bcidx: int, direction: int) -> np.ndarray:
+        """One reference differential operator applied to ``a`` at every cell."""
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.N)
+        out = np.empty(self.N)
+        rc = lib().orc_probe_grad(self._h, self.GRAD_KINDS[kind], _dp(a), int(bcidx), int(direction), _dp(out))
+        if rc != 0:
+            raise ValueError("orc_probe_grad rejected its arguments")
+        return out
+
+    def visc(self, field: int, dim: int):
+        """(nu_r, nu_l, cmax) of one field of the current state along dim."""
+        nur, nul = np.empty(self.N), np.empty(self.N)
+        cm = ctypes.c_double()
+        rc = lib().orc_probe_visc(self._h, int(field), int(dim), _dp(nur), _dp(nul), ctypes.byref(cm))
+        if rc != 0:
+            raise ValueError("orc_probe_visc rejected its arguments")
+        return nur, nul, cm.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1293,6 +1293,37 @@ int orc_get_clock(orc_t* o, int* it, double* t, double* qt, double* dt) {
   return 0;
 }
 
+/* ---------------- probes for the known-answer tests ------------------- */
+
+/* One differential operator applied to an arbitrary array at every cell:
+ * kind 0 grad3dn, 1 grad1l3n, 2 grad1r3n, 3 grad13n; bcidx selects the
+ * boundtype row that drives the boundary zeroing. */
+int orc_probe_grad(orc_t* o, int kind, const double* a, int bcidx, int dir, double* out) {
+  if (kind < 0 || kind > 3 || dir < 0 || dir >= o->nd || bcidx < 0 || bcidx >= SMAUG_MAXVAR) return -1;
+  FOR_ALL(o) {
+    const long c = enc(o, i, j, k);
+    double g;
+    if (kind == 0) g = g3dn(o, a, bcidx, i, j, k, dir);
+    else if (kind == 1) g = g1l(o, a, bcidx, i, j, k, dir);
+    else if (kind == 2) g = g1r(o, a, bcidx, i, j, k, dir);
+    else g = g13(o, a, bcidx, i, j, k, dir);
+    out[c] = g;
+  }
+  return 0;
+}
+
+/* nu_{i+1/2}, nu_{i-1/2} of one field of the current state along dim, with
+ * cmax of that dim, as the hyperdiffusion sources receive them. */
+int orc_probe_visc(orc_t* o, int field, int dim, double* nur, double* nul, double* cmax) {
+  if (field < 0 || field > F_B(o->nd - 1) || dim < 0 || dim >= o->nd) return -1;
+  o->cmax_dim[dim] = k_cmax(o, o->cur, dim);
+  k_visc(o, o->cur, field, dim);
+  memcpy(nur, WD(o, D_HDNUR), sizeof(double) * o->N);
+  memcpy(nul, WD(o, D_HDNUL), sizeof(double) * o->N);
+  if (cmax) *cmax = o->cmax_dim[dim];
+  return 0;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -14,6 +14,8 @@ int orc_set_state(orc_t* o, const double* w);
 int orc_get_wd(orc_t* o, double* wd);
 int orc_cfl(orc_t* o, double* cmax_per_dim, double* dt);
 int orc_get_clock(orc_t* o, int* it, double* t, double* qt, double* dt);
+int orc_probe_grad(orc_t* o, int kind, const double* a, int bcidx, int dir, double* out);
+int orc_probe_visc(orc_t* o, int field, int dim, double* nur, double* nul, double* cmax);
 int orc_num_threads(void);
 void orc_destroy(orc_t* o);
 #ifdef __cplusplus

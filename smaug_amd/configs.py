@@ -105,17 +105,35 @@ def hydrostatic(z: np.ndarray, rho0=2.7e-4, z0=0.1e6, grav=274.0, t_lo=6000.0, t
     return p, rho
 
 
+def stratified_dt(ni, ndim: int, courant: float, b_tesla: float = 0.01, gamma: float = 5.0 / 3.0) -> float:
+    """Fixed time step of the stratified runs: courant * min(dx) over the
+    largest fast speed sqrt((gamma p_b + B_b^2)/rho_b) of the background,
+    rounded down to 3 significant digits."""
+    xmin = [0.1e6, 0.0, 0.0][:ndim]
+    xmax = [6.0e6, 4.0e6, 4.0e6][:ndim]
+    z = _cell_centres(xmin[0], xmax[0], ni[0])
+    p, rho = hydrostatic(z)
+    bz = b_tesla / math.sqrt(MU0)
+    cmax = float(np.max(np.sqrt((gamma * p + bz * bz) / rho)))
+    dx = min((xmax[d] - xmin[d]) / ni[d] for d in range(ndim))
+    dt = courant * dx / cmax
+    e = math.floor(math.log10(dt)) - 2
+    return float(f"{math.floor(dt / 10.0 ** e)}e{e}")
+
+
 def stratified_params(ni=(128, 128, 128), ndim: int = 3, rkon: int = 1, courant: float = 0.15,
                       b_tesla: float = 0.01, amplitude: float = 200.0) -> RunParams:
     """z (dim 0) in [0.1, 6] Mm, x, y (dims 1, 2) in [0, 4] Mm, gravity -274 m/s^2
-    along dim 0, contcd4 boundaries, hyperdiffusion 0.02 (rho, e, b) and 0.2 (m)."""
+    along dim 0, contcd4 boundaries, hyperdiffusion 0.02 (rho, e, b) and 0.2 (m).
+    Fixed time step (PMODDTON 0, as models/iosmaugparams.h.fluxtube1:27), sized
+    by stratified_dt so the run is stable at the chosen resolution."""
     ni = list(ni)[:ndim]
     xmin = [0.1e6, 0.0, 0.0][:ndim]
     xmax = [6.0e6, 4.0e6, 4.0e6][:ndim]
     drv = [amplitude, 300.0, 0.1e6, 2.0e6, 2.0e6, 0.05e6, 0.1e6, 0.0]
     return RunParams(
-        ndim=ndim, ni=ni, xmin=xmin, xmax=xmax, dt=1.0, gamma=5.0 / 3.0,
-        g=[-274.0, 0.0, 0.0], courant=courant, rkon=rkon, moddton=1, hyperdifmom=1,
+        ndim=ndim, ni=ni, xmin=xmin, xmax=xmax, dt=stratified_dt(ni, ndim, courant, b_tesla),
+        gamma=5.0 / 3.0, g=[-274.0, 0.0, 0.0], courant=courant, rkon=rkon, moddton=0, hyperdifmom=1,
         chyp={"default": 0.02, "rho": 0.02, "energy": 0.02, "b1": 0.02, "b2": 0.02, "b3": 0.02,
               "mom1": 0.2, "mom2": 0.2, "mom3": 0.2},
         bcu=[BC_CONTCD4] * 3, bcl=[BC_CONTCD4] * 3, driver=DRIVER_VERTICAL, drv=drv)

@@ -43,16 +43,13 @@ struct Lay {
   // q box: tile + 3-cell halo
   static constexpr int QX = TX + 6, QY = TY + 6, QZ = ND == 3 ? TZ + 6 : 1;
   static constexpr int QN = QX * QY * QZ;
-  // aux box: tile + 1-cell halo (rho_t, m_0..m_{nd-1})
-  static constexpr int AX = TX + 2, AY = TY + 2, AZ = ND == 3 ? TZ + 2 : 1;
-  static constexpr int AN = AX * AY * AZ;
   // region: tile extended by one cell along one axis
   static constexpr int RN = ND == 3 ? imax(imax((TX + 2) * TY * TZ, TX * (TY + 2) * TZ), TX * TY * (TZ + 2))
                                     : imax((TX + 2) * TY, TX * (TY + 2));
   // flux region: tile extended by two cells along one axis
   static constexpr int FN = ND == 3 ? imax(imax((TX + 4) * TY * TZ, TX * (TY + 4) * TZ), TX * TY * (TZ + 4))
                                     : imax((TX + 4) * TY, TX * (TY + 4));
-  static constexpr int HYP = QN + (1 + ND) * AN + 4 * RN;
+  static constexpr int HYP = QN + 4 * RN;
   static constexpr int FLX = NP * FN;
   static constexpr int SMEM = HYP > FLX ? HYP : FLX;
 };
@@ -200,7 +197,7 @@ __device__ inline void fluxes2(const Geo& G, const double* __restrict__ w, const
 }
 
 template <int ND>
-__global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
   constexpr int NP = L::NP;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
@@ -279,39 +276,20 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
 
   // ------------------------------------------------------------ hyperdiffusion + energy terms
   double* qb = sm;
-  double* ax = sm + L::QN;            // (1+ND) * AN: rho_t, m_0..
-  double* NR = ax + (1 + ND) * L::AN; // region arrays
+  double* NR = sm + L::QN;  // region arrays
   double* NL = NR + L::RN;
   double* GA = NL + L::RN;
   double* GB = GA + L::RN;
-  // tile-cell indices in the q box and the aux box
+  // tile-cell index in the q box and grid strides
   const int qc = (lc[0] + 3) + L::QX * ((lc[1] + 3) + L::QY * (ND == 3 ? lc[2] + 3 : 0));
   const int qs[3] = {1, L::QX, L::QX * L::QY};
-  const int ac = (lc[0] + 1) + L::AX * ((lc[1] + 1) + L::AY * (ND == 3 ? lc[2] + 1 : 0));
-  const int as[3] = {1, L::AX, L::AX * L::AY};
+  const long long gs[3] = {1, G.n[0], (long long)G.n[0] * G.n[1]};
 
   const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
   const bool want_eterm = (T & SMAUG_T_ETERM) != 0;
   double cm[ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) cm[d] = __longlong_as_double((long long)A.cmax_bits[d]);
-
-  if (want_mom) {  // rho_t and momenta with a 1-cell halo
-    for (int l = tid; l < L::AN; l += 256) {
-      const int a = l % L::AX, b = (l / L::AX) % L::AY, cz = l / (L::AX * L::AY);
-      int xa[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
-      if (in_grid<ND>(G, xa)) {
-        const long long g = gix(G, xa[0], xa[1], xa[2]);
-        ax[l] = w[g] + bg[(long long)bRHOB() * N + g];
-#pragma unroll
-        for (int q = 0; q < ND; q++) ax[(1 + q) * L::AN + l] = w[(long long)fMOM(q) * N + g];
-      } else {
-        ax[l] = 1.0;
-#pragma unroll
-        for (int q = 0; q < ND; q++) ax[(1 + q) * L::AN + l] = 0.0;
-      }
-    }
-  }
 
   if (G.hyper && (T & SMAUG_T_HRHO)) {  // src/hyperdifrhosource1.cu
     stage_q<ND>(G, w, qb, o);
@@ -396,15 +374,14 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
               double nr = 0, nl = 0, g7 = 0, g8 = 0;
               if (interior_c<ND>(G, xr)) {
                 const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
-                const int ai = (a - ex[0] + 1) + L::AX * ((b - ex[1] + 1) + L::AY * (ND == 3 ? cz - ex[2] + 1 : 0));
                 const long long gq = gix(G, xr[0], xr[1], xr[2]);
                 const double dxr = A.delx[(long long)dim * N + gq], rdxr = A.rdelx[(long long)dim * N + gq];
                 nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
                 const double t5 =
                     z1(G, zTMP4, dim, xr[dim]) ? 0.0 : (qb[qi + qs[dim]] - qb[qi - qs[dim]]) * rdxr * 0.5;
                 const double t6 = t5 * (nr + nl) * 0.25;
-                g7 = ax[ai] * t6;
-                g8 = t6 * ax[(1 + dim) * L::AN + ai];
+                g7 = (w[gq] + bg[(long long)bRHOB() * N + gq]) * t6;
+                g8 = t6 * w[(long long)fMOM(dim) * N + gq];
               }
               NR[l] = nr;
               NL[l] = nl;
@@ -420,14 +397,17 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
             nu_win(qb, qc, qs[dim], dx[dim] * K, nur, nul);
           }
           if (valid) {
-            const int s = qs[dim], sa = as[dim];
+            const int s = qs[dim];
+            const long long sg = gs[dim];
             const bool z = z1(G, zTMP4, dim, x[dim]);
             const double q0 = qb[qc];
             const double gr = z ? 0.0 : (qb[qc + s] - q0) * rdx[dim];
             const double gl = z ? 0.0 : (q0 - qb[qc - s]) * rdx[dim];
-            const double rr = (ax[ac] + ax[ac + sa]) * 0.5, rl = (ax[ac] + ax[ac - sa]) * 0.5;
-            const double* mf = ax + (1 + f) * L::AN;
-            const double mr = (mf[ac] + mf[ac + sa]) * 0.5, ml = (mf[ac] + mf[ac - sa]) * 0.5;
+            const double* rb = bg + (long long)bRHOB() * N;
+            const double rtp = w[c + sg] + rb[c + sg], rtm = w[c - sg] + rb[c - sg];
+            const double rr = (rt_c + rtp) * 0.5, rl = (rt_c + rtm) * 0.5;
+            const double* mf = w + (long long)fMOM(f) * N;
+            const double mr = (mf[c] + mf[c + sg]) * 0.5, ml = (mf[c] + mf[c - sg]) * 0.5;
             const double dmv = (rr * nur * gr - rl * nul * gl) * (rdx[dim] * 0.5);
             const double dev = (mr * nur * gr - ml * nul * gl) * (rdx[dim] * 0.5);
             acc[fMOM(f)] += dmv;

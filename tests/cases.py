@@ -17,6 +17,13 @@ def strat_case(ni=(14, 10, 12), ndim=3, rkon=0, seed=20261202, driver=True):
     rp = configs.stratified_params(ni=ni[:ndim], ndim=ndim, rkon=rkon)
     if not driver:
         rp.driver = 0
+    else:
+        # driver footprint scaled to the small grid (a few cells wide, centred
+        # near the bottom) and phase near its maximum, so the source term is
+        # exercised at full strength rather than through exp(-35) tails
+        dx = rp.dx
+        rp.drv = [200.0, 300.0, rp.xmin[0] + 2.5 * dx[0], 2.0e6, 2.0e6, 3.0 * dx[0], 3.0 * dx[1], 0.0]
+        rp.qt0 = 75.0
     w, pos = configs.stratified(rp, perturb=seed, amp=1e-3)
     return rp, w, pos
 
