@@ -68,6 +68,21 @@ def cpu_baseline(size: int, steps: int):
     }
 
 
+COPY_PROBE_BYTES = 256 << 20
+
+
+def copy_probe(torch):
+    """A plain copy of known size (reads and writes COPY_PROBE_BYTES each):
+    under rocprofv3 --pmc it gives the scale of FETCH_SIZE / WRITE_SIZE."""
+    src = torch.rand(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: 3 x {COPY_PROBE_BYTES} B read + written", file=sys.stderr, flush=True)
+    del src, dst
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,8 +91,10 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=256, help="interior cells per dim")
     ap.add_argument("--kernel-steps", type=int, default=3, help="steps timed per kernel with events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-size", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-size", type=int, default=96)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a plain 256 MiB -> 256 MiB elementwise copy first (HBM counter calibration)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -94,6 +111,8 @@ def main(argv=None):
     from smaug_amd import configs
     from smaug_amd.engine import Engine
 
+    if args.copy_probe:
+        copy_probe(torch)
     S = args.size
     rp = configs.stratified_params(ni=(S, S, S), rkon=1)
     w, pos = configs.stratified(rp, perturb=20261202)

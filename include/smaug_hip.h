@@ -92,7 +92,11 @@ typedef struct smaug_params {
   double dx[3];             /* p->dx = (xmax-xmin)/NI, used by the CFL rule  */
   int boundtype[SMAUG_MAXVAR][3][2]; /* [field][dir][0=upper BCU,1=lower BCL] */
   int driver;               /* SMAUG_DRIVER_*                                */
-  double drv[8];            /* driver parameters             */
+  double drv[8];            /* SMAUG_DRIVER_VERTICAL: A [m/s], period P [s], z0,
+                               x0, y0 [m], sigma_z, sigma_xy [m], unused;
+                               v_z = A exp(-dz^2/sz^2 - dx^2/sxy^2 [- dy^2/sxy^2])
+                               sin(2 pi qt / P), m_0 += dt v_z rho_t,
+                               e += dt v_z^2 rho_t / 2                       */
   unsigned terms;           /* SMAUG_T_* mask, SMAUG_TERMS_ALL normally      */
   int precision;            /* SMAUG_PREC_*                                  */
   double qt0;               /* initial value of the nominal clock p->qt      */
