@@ -146,6 +146,25 @@ int smaug_destroy(smaug_t* s);
 const char* smaug_last_error(void);
 const char* smaug_version(void);
 
+/* VAC configuration files (host only, no device needed).
+ *   smaug_ini_read   <- readasciivacconfig (src/readwrite.c:975-1082): reads
+ *                       an ASCII .ini of ndim dims n[] (ghosts included) into
+ *                       w (NVAR*N) and pos (ndim*N); the header must match.
+ *   smaug_ini_write  <- writeasciivacconfig (src/readwrite.c:1085-1300),
+ *                       same header and %lG rows (6 significant digits).
+ *   smaug_out_write  <- writevacconfig (src/readwrite.c:160-366): binary
+ *                       snapshot, 79-byte name, it, t, ndim, neqpar, nw, n[],
+ *                       eqpar[], 79-byte variable names, positions and
+ *                       fields as doubles (i fastest), trailing '\n'.
+ *   smaug_write_out  <- the call in the time loop (src/iosmaug.c:517): the
+ *                       context's current state to <prefix>_<it>.out. */
+int smaug_ini_read(const char* path, int ndim, const int* n, double* w, double* pos, int* it, double* t);
+int smaug_ini_write(const char* path, const char* name, int ndim, const int* n, const double* w,
+                    const double* pos, int it, double t, double gamma, double eta, const double* g);
+int smaug_out_write(const char* path, const char* name, int ndim, const int* n, const double* w,
+                    const double* pos, int it, double t, double gamma, double eta, const double* g);
+int smaug_write_out(smaug_t* s, const char* prefix, const char* name);
+
 /* Kernel timing hooks for the benchmark: launches `reps` RK stages of the
  * configured step on the current state and reports the mean device time of
  * each kernel family measured with HIP events on the context's stream. */

@@ -17,15 +17,31 @@ using namespace smaug;
 static thread_local std::string g_err;
 
 #include <stdarg.h>
-static int fail(int code, const char* fmt, ...) {
+static int vfail(int code, const char* fmt, va_list ap) {
   char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
   g_err = buf;
   return code;
 }
+
+static int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  const int r = vfail(code, fmt, ap);
+  va_end(ap);
+  return r;
+}
+
+namespace smaug {
+// error reporting for the file readers/writers (smaug_io.cpp)
+int io_fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  const int r = vfail(code, fmt, ap);
+  va_end(ap);
+  return r;
+}
+}  // namespace smaug
 
 #define HIPCHK(call)                                                                   \
   do {                                                                                 \
@@ -405,6 +421,21 @@ extern "C" int smaug_get_grid(const smaug_t* s, double* pos, double* delx) {
   if (pos) memcpy(pos, s->h_pos.data(), sizeof(double) * s->h_pos.size());
   if (delx) memcpy(delx, s->h_delx.data(), sizeof(double) * s->h_delx.size());
   return SMAUG_OK;
+}
+
+extern "C" int smaug_write_out(smaug_t* s, const char* prefix, const char* name) {
+  if (!s || !prefix) return fail(SMAUG_E_ARG, "null argument");
+  std::vector<double> w((size_t)(s->np + s->nb) * s->N);
+  int rc = smaug_get_state(s, w.data());
+  if (rc != SMAUG_OK) return rc;
+  int it = 0;
+  double t = 0;
+  rc = smaug_get_clock(s, &it, &t, nullptr, nullptr);
+  if (rc != SMAUG_OK) return rc;
+  // src/readwrite.c:222: <name>_<it>.out
+  std::string path = std::string(prefix) + "_" + std::to_string(it) + ".out";
+  return smaug_out_write(path.c_str(), name ? name : "", s->nd, s->p.n, w.data(), s->h_pos.data(), it, t,
+                         s->p.gamma, s->p.eta, s->p.g);
 }
 
 extern "C" int smaug_get_clock(const smaug_t* cs, int* it, double* t, double* qt, double* dt_last) {

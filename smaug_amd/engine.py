@@ -22,6 +22,7 @@ EXPORTS = [
     "smaug_create", "smaug_step", "smaug_cfl", "smaug_fill_ghosts", "smaug_get_state",
     "smaug_set_state", "smaug_get_grid", "smaug_get_clock", "smaug_state_device_ptr",
     "smaug_sync", "smaug_destroy", "smaug_last_error", "smaug_version", "smaug_time_kernels",
+    "smaug_ini_read", "smaug_ini_write", "smaug_out_write", "smaug_write_out",
 ]
 
 
@@ -52,6 +53,12 @@ def load() -> ctypes.CDLL:
     L.smaug_last_error.restype = ctypes.c_char_p
     L.smaug_version.restype = ctypes.c_char_p
     L.smaug_time_kernels.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    cp = ctypes.c_char_p
+    L.smaug_ini_read.argtypes = [cp, ctypes.c_int, ip, dp, dp, ip, dp]
+    L.smaug_ini_write.argtypes = [cp, cp, ctypes.c_int, ip, dp, dp, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, dp]
+    L.smaug_out_write.argtypes = L.smaug_ini_write.argtypes
+    L.smaug_write_out.argtypes = [vp, cp, cp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype if name in (
             "smaug_last_error", "smaug_version") else ctypes.c_int
@@ -125,6 +132,11 @@ class Engine:
         p = ctypes.c_void_p()
         _chk(load().smaug_state_device_ptr(self._h, ctypes.byref(p)), "smaug_state_device_ptr")
         return p.value
+
+    def write_out(self, prefix: str, name: str = "") -> str:
+        """Binary VAC snapshot of the current state: <prefix>_<it>.out."""
+        _chk(load().smaug_write_out(self._h, prefix.encode(), name.encode()), "smaug_write_out")
+        return f"{prefix}_{self.clock()[0]}.out"
 
     def time_kernels(self, nsteps: int) -> Tuple[float, float, float]:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
