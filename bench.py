@@ -17,6 +17,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import re
 import sys
@@ -110,6 +112,7 @@ def main(argv=None):
         return 1
     from smaug_amd import configs
     from smaug_amd.engine import Engine
+    from smaug_amd.params import PREC_MIXED, field_names
 
     if args.copy_probe:
         copy_probe(torch)
@@ -117,24 +120,55 @@ def main(argv=None):
     rp = configs.stratified_params(ni=(S, S, S), rkon=1)
     w, pos = configs.stratified(rp, perturb=20261202)
     print(f"[bench] state built: {S}^3 interior, {rp.ncells} cells incl. ghosts", file=sys.stderr, flush=True)
-    eng = Engine(rp, w, pos)
-    del w, pos
-    eng.step(args.warmup)
-    eng.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.step(args.steps)
-    eng.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
     cells = S ** 3
+
+    def timed(eng, label):
+        eng.step(args.warmup)
+        eng.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.step(args.steps)
+        eng.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        print(f"[bench] {label}: timed {args.steps} steps, {wall / args.steps * 1e3:.3f} ms/step",
+              file=sys.stderr, flush=True)
+        return wall
+
+    # FP64 (the plain form): the headline value
+    eng = Engine(rp, w, pos)
+    wall = timed(eng, "fp64")
     value = cells * args.steps / wall
     ms_step = wall / args.steps * 1e3
-    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
-
+    state64 = eng.state()  # before the event-timed steps, to compare with the mixed run
     ms_prim, ms_rhs, ms_evstep = eng.time_kernels(args.kernel_steps)
     eng.close()
+
+    # mixed precision: FP32 stencil shadows, FP64 state/CFL/combine, same
+    # initial state and step count; error against the FP64 run
+    rpm = configs.stratified_params(ni=(S, S, S), rkon=1)
+    rpm.precision = PREC_MIXED
+    engm = Engine(rpm, w, pos)
+    del w, pos
+    wallm = timed(engm, "mixed")
+    statem = engm.state()
+    msm_prim, msm_rhs, _ = engm.time_kernels(args.kernel_steps)
+    engm.close()
+    k = 2 + 2 * rp.ndim
+    num = np.sqrt(((statem[:k] - state64[:k]) ** 2).sum(axis=1))
+    den = np.sqrt((state64[:k] ** 2).sum(axis=1))
+    rel = (num / np.where(den == 0, 1.0, den)).tolist()
+    del state64, statem
+    mixed = {
+        "value": cells * args.steps / wallm,
+        "unit": "cell-updates/s",
+        "ms_per_step": wallm / args.steps * 1e3,
+        "speedup_vs_fp64": wall / wallm,
+        "k_rhs_ms_per_launch": msm_rhs,
+        "k_prim_ms_per_launch": msm_prim,
+        "rel_l2_vs_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], rel)),
+        "steps_compared": args.warmup + args.steps,
+    }
     rhs_bytes = RHS_BYTES_PER_CELL * cells
     achieved = rhs_bytes / (ms_rhs * 1e-3) / 1e9
     roof = {
@@ -175,6 +209,7 @@ def main(argv=None):
             "precision": "fp64",
             "scheme": "rk4",
         },
+        "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

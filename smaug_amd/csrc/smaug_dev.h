@@ -33,7 +33,7 @@ struct Geo {
   int hyper;             // PHYPERDIFMOM
   int driver;
   double drv[8];
-  int kver;              // k_rhs variant: 2 tiled (default), 1 per-cell
+  int mixed;             // 1: stencils read FP32 shadows (SMAUG_PREC_MIXED)
 };
 
 // evolved-field indices
@@ -133,9 +133,11 @@ __host__ __device__ inline long long halo_r_src(const Geo& G, const int* pc) {
 }
 
 struct RhsArgs {
-  const double* win;    // stage input, evolved fields
+  const double* win;    // stage input, evolved fields (FP64, own-cell values)
+  const void* wsh;      // stage input as the stencils read it: win itself
+                        // (FP64) or its FP32 shadow written by k_prim (mixed)
   const double* bg;     // background
-  const double* prim;   // k_prim output of win
+  const void* prim;     // k_prim output of win: double (FP64) or float (mixed)
   const double* delx;   // nd*N
   const double* rdelx;  // nd*N, 1/delx
   const double* pos;    // nd*N
@@ -151,7 +153,9 @@ struct RhsArgs {
   double tdep;                          // driver time factor sin(2 pi qt/P)
 };
 
-void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg, double* prim,
+// k_prim: prim (double, or float when mixed) and, when mixed, the FP32
+// shadow of the evolved fields of w
+void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg, void* prim, float* shadow,
                  unsigned long long* cmax, int* flag);
 void launch_zero3(hipStream_t st, unsigned long long* p);
 void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,

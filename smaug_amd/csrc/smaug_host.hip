@@ -57,7 +57,8 @@ struct smaug_ctx {
   long long N = 0;
   double* U[4] = {nullptr, nullptr, nullptr, nullptr};  // evolved-field buffers
   double* bg = nullptr;
-  double* prim = nullptr;
+  void* prim = nullptr;     // double, or float when mixed
+  float* shadow = nullptr;  // mixed: FP32 copy of the stage input for the stencils
   double* delx = nullptr;
   double* rdelx = nullptr;
   double* pos = nullptr;
@@ -191,8 +192,7 @@ static void make_geo(const smaug_params* p, Geo& G) {
   G.hyper = p->hyperdifmom;
   G.driver = p->driver;
   for (int i = 0; i < 8; i++) G.drv[i] = p->drv[i];
-  const char* kv = getenv("SMAUG_RHS");
-  G.kver = (kv && kv[0] == '1') ? 1 : 2;
+  G.mixed = p->precision == SMAUG_PREC_MIXED;
 }
 
 static void free_ctx(smaug_t* s) {
@@ -201,6 +201,7 @@ static void free_ctx(smaug_t* s) {
     if (u) hipFree(u);
   hipFree(s->bg);
   hipFree(s->prim);
+  hipFree(s->shadow);
   hipFree(s->delx);
   hipFree(s->rdelx);
   hipFree(s->pos);
@@ -215,8 +216,6 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   if (!p || !w0 || !pos || !out) return fail(SMAUG_E_ARG, "null argument");
   int rc = check_params(p);
   if (rc) return rc;
-  if (p->precision != SMAUG_PREC_FP64)
-    return fail(SMAUG_E_UNSUPPORTED, "mixed precision is not built in this library version");
   smaug_t* s = new smaug_t();
   s->p = *p;
   make_geo(p, s->G);
@@ -236,7 +235,9 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   bool ok = true;
   for (int b = 0; b < nbuf; b++) ok &= hipMalloc(&s->U[b], bytes * s->np) == hipSuccess;
   ok &= hipMalloc(&s->bg, bytes * s->nb) == hipSuccess;
-  ok &= hipMalloc(&s->prim, bytes * s->npr) == hipSuccess;
+  const size_t pbytes = (s->G.mixed ? sizeof(float) : sizeof(double)) * (size_t)s->N;
+  ok &= hipMalloc(&s->prim, pbytes * s->npr) == hipSuccess;
+  if (s->G.mixed) ok &= hipMalloc(&s->shadow, sizeof(float) * (size_t)s->N * s->np) == hipSuccess;
   ok &= hipMalloc(&s->delx, bytes * s->nd) == hipSuccess;
   ok &= hipMalloc(&s->rdelx, bytes * s->nd) == hipSuccess;
   ok &= hipMalloc(&s->pos, bytes * s->nd) == hipSuccess;
@@ -281,6 +282,7 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
                         double tdep, int slot) {
   RhsArgs A;
   A.win = win;
+  A.wsh = s->G.mixed ? (const void*)s->shadow : (const void*)win;
   A.bg = s->bg;
   A.prim = s->prim;
   A.delx = s->delx;
@@ -301,7 +303,7 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
 static void run_stage(smaug_t* s, const double* win, const double* base, double* wout, double hfrac, double tdep,
                       bool first, const double* r1, const double* r2, const double* r3) {
   const int slot = s->cslot;
-  launch_prim(s->G, s->stream, win, s->bg, s->prim, s->cmax + 4 * slot, s->flag);
+  launch_prim(s->G, s->stream, win, s->bg, s->prim, s->shadow, s->cmax + 4 * slot, s->flag);
   if (first)
     launch_dt(s->stream, s->cmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->dts);
   RhsArgs A = rhs_args(s, win, base, wout, hfrac, tdep, slot);
@@ -364,7 +366,7 @@ extern "C" int smaug_cfl(smaug_t* s, double* cmax_per_dim, double* dt) {
   if (!s) return fail(SMAUG_E_ARG, "null context");
   unsigned long long* slot = s->cmax + 8;
   launch_zero3(s->stream, slot);
-  launch_prim(s->G, s->stream, s->U[s->cur], s->bg, s->prim, slot, s->flag);
+  launch_prim(s->G, s->stream, s->U[s->cur], s->bg, s->prim, s->shadow, slot, s->flag);
   unsigned long long bits[3];
   double dts[2];
   HIPCHK(hipMemcpyAsync(bits, slot, sizeof bits, hipMemcpyDeviceToHost, s->stream));
@@ -500,7 +502,7 @@ extern "C" int smaug_time_kernels(smaug_t* s, int nsteps, double* ms_prim, doubl
       }
       const int slot = s->cslot;
       HIPCHK(hipEventRecord(ev[4 * st + 0], s->stream));
-      launch_prim(s->G, s->stream, win, s->bg, s->prim, s->cmax + 4 * slot, s->flag);
+      launch_prim(s->G, s->stream, win, s->bg, s->prim, s->shadow, s->cmax + 4 * slot, s->flag);
       HIPCHK(hipEventRecord(ev[4 * st + 1], s->stream));
       if (st == 0)
         launch_dt(s->stream, s->cmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->dts);

@@ -1,6 +1,6 @@
 // smaug_rhs.hip -- tiled right-hand side of one SMAUG stage (k_rhs).
 //
-// One workgroup owns a tile of interior cells (8x8x4 in 3D, 16x16 in 2D,
+// One workgroup owns a tile of interior cells (8x4x8 in 3D, 16x16 in 2D,
 // 256 threads, one cell each) and builds every term of the reference step
 // for it from LDS-staged data, so each intermediate is computed once:
 //   1. flux phase, per direction d: the eight fluxes F_f^d of the tile
@@ -16,6 +16,9 @@
 //      background energy terms, which need gradients of v_f only.
 //   3. gravity, the stage update, the user source, the RK combination and
 //      the ghost-cell images (boundaries fused).
+// Neighbour values are read in the stencil precision SH: the FP64 state
+// itself, or (mixed precision) the FP32 shadows k_prim writes; all
+// arithmetic, the own-cell values and the update are FP64.
 #include <math.h>
 
 #include "../../include/smaug_hip.h"
@@ -25,9 +28,13 @@ namespace smaug {
 
 template <int ND>
 struct Tile;
+// 3D tile 8 x 4 x 8: dim 0 (contiguous) 8 wide for coalesced rows; the
+// workgroups resident on one XCD cover whole x-rows and neighbouring
+// y-rows, so the tile is deep in z, the direction whose neighbour tiles are
+// least likely to still be in L2 (fewer halo planes re-fetched per cell)
 template <>
 struct Tile<3> {
-  static constexpr int X = 8, Y = 8, Z = 4;
+  static constexpr int X = 8, Y = 4, Z = 8;
 };
 template <>
 struct Tile<2> {
@@ -77,8 +84,8 @@ __device__ inline bool interior_c(const Geo& G, const int* x) {
 // q at grid coordinates x, one step past the grid allowed in one direction:
 // coordinate -1 reads coordinate 5 (left-chain halo, quirk Q8), coordinate n
 // reads the right-chain stray halo (quirk Q7).
-template <int ND>
-__device__ inline double qval(const Geo& G, const double* __restrict__ qa, const int* x) {
+template <int ND, typename SH>
+__device__ inline SH qval(const Geo& G, const SH* __restrict__ qa, const int* x) {
   int out = -1, nout = 0;
 #pragma unroll
   for (int d = 0; d < ND; d++)
@@ -87,7 +94,7 @@ __device__ inline double qval(const Geo& G, const double* __restrict__ qa, const
       nout++;
     }
   if (nout == 0) return qa[gix(G, x[0], x[1], ND == 3 ? x[2] : 0)];
-  if (nout > 1) return 0.0;
+  if (nout > 1) return (SH)0;
   int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
   if (x[out] == -1) {
     y[out] = 5;
@@ -99,16 +106,16 @@ __device__ inline double qval(const Geo& G, const double* __restrict__ qa, const
     for (int q = 0; q < 3; q++) pc[q] = y[q] + 1;
     pc[out] = G.n[out] + 1;
     const long long h = halo_r_src(G, pc);
-    return h < 0 ? 0.0 : qa[h];
+    return h < 0 ? (SH)0 : qa[h];
   }
-  return 0.0;
+  return (SH)0;
 }
 
 // nu_{i+1/2}, nu_{i-1/2} from a 7-cell window of q in LDS (centre index ci,
 // stride s): third differences at 4 interfaces, first differences at 6,
 // maxima over 3 and 5 interfaces (src/hyperdifvisc1r.cu:571-565 chain).
-__device__ inline void nu_win(const double* __restrict__ qb, int ci, int s, double dxk, double& nur,
-                              double& nul) {
+template <typename SH>
+__device__ inline void nu_win(const SH* __restrict__ qb, int ci, int s, double dxk, double& nur, double& nul) {
   double Q[7];
 #pragma unroll
   for (int o = 0; o < 7; o++) Q[o] = qb[ci + (o - 3) * s];
@@ -127,9 +134,8 @@ __device__ inline void nu_win(const double* __restrict__ qb, int ci, int s, doub
   nul = m5l > 0 ? dxk * m3l / m5l : 0.0;
 }
 
-template <int ND>
-__device__ inline void stage_q(const Geo& G, const double* __restrict__ qa, double* __restrict__ qb,
-                               const int* o) {
+template <int ND, typename SH>
+__device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o) {
   using L = Lay<ND>;
   // fast path: the halo box lies inside the grid, no substitutions needed;
   // each thread owns one (a,b) column of the box and walks its planes
@@ -140,7 +146,7 @@ __device__ inline void stage_q(const Geo& G, const double* __restrict__ qa, doub
     const long long zs = (long long)G.n[0] * G.n[1];
     for (int l = threadIdx.x; l < PL; l += 256) {
       const int a = l % L::QX, b = l / L::QX;
-      const double* src = qa + gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
+      const SH* src = qa + gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
 #pragma unroll
       for (int cz = 0; cz < L::QZ; cz++) qb[cz * PL + l] = src[cz * zs];
     }
@@ -149,15 +155,15 @@ __device__ inline void stage_q(const Geo& G, const double* __restrict__ qa, doub
   for (int l = threadIdx.x; l < L::QN; l += 256) {
     const int a = l % L::QX, b = (l / L::QX) % L::QY, c = l / (L::QX * L::QY);
     int x[3] = {o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 + c : 0};
-    qb[l] = qval<ND>(G, qa, x);
+    qb[l] = qval<ND, SH>(G, qa, x);
   }
 }
 
 // fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,
 // src/centdiff2.cu:158-356), from v = m/rho_t and p_t of k_prim
-template <int ND>
-__device__ inline void fluxes2(const Geo& G, const double* __restrict__ w, const double* __restrict__ bg,
-                               const double* __restrict__ pr, long long cc, int d, double* F) {
+template <int ND, typename SH>
+__device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg,
+                               const SH* __restrict__ pr, long long cc, int d, double* F) {
   const long long N = G.N;
   double v[ND], b[ND], bb[ND];
 #pragma unroll
@@ -196,7 +202,7 @@ __device__ inline void fluxes2(const Geo& G, const double* __restrict__ w, const
     F[fB(ND, f)] = (f == d) ? 0.0 : vd * b[f] - (bd + bbd) * v[f] + bb[f] * vd;
 }
 
-template <int ND>
+template <int ND, typename SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
   constexpr int NP = L::NP;
@@ -221,9 +227,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const bool valid = interior_c<ND>(G, x);
   const long long N = G.N;
   const long long c = gix(G, x[0], x[1], x[2]);
-  const double* __restrict__ w = A.win;
+  const double* __restrict__ w = A.win;           // own-cell values (FP64)
+  const SH* __restrict__ ws = (const SH*)A.wsh;    // neighbour values (stencil precision)
   const double* __restrict__ bg = A.bg;
-  const double* __restrict__ pr = A.prim;
+  const SH* __restrict__ pr = (const SH*)A.prim;
   const unsigned T = G.terms;
   double acc[NP];
 #pragma unroll
@@ -247,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
         double F[NP];
         if (in_grid<ND>(G, xr)) {
-          fluxes2<ND>(G, w, bg, pr, gix(G, xr[0], xr[1], xr[2]), d, F);
+          fluxes2<ND, SH>(G, ws, bg, pr, gix(G, xr[0], xr[1], xr[2]), d, F);
         } else {
 #pragma unroll
           for (int f = 0; f < NP; f++) F[f] = 0.0;
@@ -275,8 +282,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const double rt_c = rho_c + rhob_c;
 
   // ------------------------------------------------------------ hyperdiffusion + energy terms
-  double* qb = sm;
-  double* NR = sm + L::QN;  // region arrays
+  SH* qb = reinterpret_cast<SH*>(sm);                                        // q box
+  double* NR = sm + (L::QN * sizeof(SH) + sizeof(double) - 1) / sizeof(double);  // region arrays
   double* NL = NR + L::RN;
   double* GA = NL + L::RN;
   double* GB = GA + L::RN;
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int d = 0; d < ND; d++) cm[d] = __longlong_as_double((long long)A.cmax_bits[d]);
 
   if (G.hyper && (T & SMAUG_T_HRHO)) {  // src/hyperdifrhosource1.cu
-    stage_q<ND>(G, w, qb, o);
+    stage_q<ND, SH>(G, ws, qb, o);
     __syncthreads();
     if (valid) {
 #pragma unroll
@@ -300,16 +307,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         double nur, nul;
         nu_win(qb, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fRHO()], nur, nul);
         const bool z = z1(G, zRHO, dim, x[dim]);
-        const double q0 = qb[qc];
-        const double gr = z ? 0.0 : (qb[qc + qs[dim]] - q0) * rdx[dim];
-        const double gl = z ? 0.0 : (q0 - qb[qc - qs[dim]]) * rdx[dim];
+        const double q0 = (double)qb[qc];
+        const double gr = z ? 0.0 : ((double)qb[qc + qs[dim]] - q0) * rdx[dim];
+        const double gl = z ? 0.0 : (q0 - (double)qb[qc - qs[dim]]) * rdx[dim];
         acc[fRHO()] += (nur * gr - nul * gl) * rdx[dim];
       }
     }
     __syncthreads();
   }
   if (G.hyper && (T & SMAUG_T_HE)) {  // src/hyperdifesource1.cu
-    stage_q<ND>(G, pr + (long long)pQE(ND) * N, qb, o);
+    stage_q<ND, SH>(G, pr + (long long)pQE(ND) * N, qb, o);
     __syncthreads();
     if (valid) {
 #pragma unroll
@@ -317,9 +324,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         double nur, nul;
         nu_win(qb, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fE(ND)], nur, nul);
         const bool z = z1(G, zTMP1, dim, x[dim]);
-        const double q0 = qb[qc];
-        const double gr = z ? 0.0 : (qb[qc + qs[dim]] - q0) * rdx[dim];
-        const double gl = z ? 0.0 : (q0 - qb[qc - qs[dim]]) * rdx[dim];
+        const double q0 = (double)qb[qc];
+        const double gr = z ? 0.0 : ((double)qb[qc + qs[dim]] - q0) * rdx[dim];
+        const double gl = z ? 0.0 : (q0 - (double)qb[qc - qs[dim]]) * rdx[dim];
         acc[fE(ND)] += (nur * gr - nul * gl) * rdx[dim];
       }
     }
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (want_mom || want_eterm) {
 #pragma unroll
     for (int f = 0; f < ND; f++) {
-      stage_q<ND>(G, pr + (long long)pV(f) * N, qb, o);
+      stage_q<ND, SH>(G, pr + (long long)pV(f) * N, qb, o);
       __syncthreads();
       if (want_eterm && valid) {  // addenergyterms_cd2 for dir f
         double gf[ND];
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           const int s = qs[i2];
           gf[i2] = z4(G, zVEL(f), i2, x[i2])
                        ? 0.0
-                       : ((8 * qb[qc + s] - 8 * qb[qc - s]) + (qb[qc - 2 * s] - qb[qc + 2 * s])) * (rdx[i2] * (1.0 / 12.0));
+                       : ((8 * (double)qb[qc + s] - 8 * (double)qb[qc - s]) + ((double)qb[qc - 2 * s] - (double)qb[qc + 2 * s])) * (rdx[i2] * (1.0 / 12.0));
         }
         double t = -ptb * gf[f];
 #pragma unroll
@@ -378,10 +385,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const double dxr = A.delx[(long long)dim * N + gq], rdxr = A.rdelx[(long long)dim * N + gq];
                 nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
                 const double t5 =
-                    z1(G, zTMP4, dim, xr[dim]) ? 0.0 : (qb[qi + qs[dim]] - qb[qi - qs[dim]]) * rdxr * 0.5;
+                    z1(G, zTMP4, dim, xr[dim]) ? 0.0 : ((double)qb[qi + qs[dim]] - (double)qb[qi - qs[dim]]) * rdxr * 0.5;
                 const double t6 = t5 * (nr + nl) * 0.25;
-                g7 = (w[gq] + bg[(long long)bRHOB() * N + gq]) * t6;
-                g8 = t6 * w[(long long)fMOM(dim) * N + gq];
+                g7 = ((double)ws[gq] + bg[(long long)bRHOB() * N + gq]) * t6;
+                g8 = t6 * (double)ws[(long long)fMOM(dim) * N + gq];
               }
               NR[l] = nr;
               NL[l] = nl;
@@ -400,14 +407,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const int s = qs[dim];
             const long long sg = gs[dim];
             const bool z = z1(G, zTMP4, dim, x[dim]);
-            const double q0 = qb[qc];
-            const double gr = z ? 0.0 : (qb[qc + s] - q0) * rdx[dim];
-            const double gl = z ? 0.0 : (q0 - qb[qc - s]) * rdx[dim];
+            const double q0 = (double)qb[qc];
+            const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[dim];
+            const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[dim];
             const double* rb = bg + (long long)bRHOB() * N;
-            const double rtp = w[c + sg] + rb[c + sg], rtm = w[c - sg] + rb[c - sg];
+            const double rtp = (double)ws[c + sg] + rb[c + sg], rtm = (double)ws[c - sg] + rb[c - sg];
             const double rr = (rt_c + rtp) * 0.5, rl = (rt_c + rtm) * 0.5;
-            const double* mf = w + (long long)fMOM(f) * N;
-            const double mr = (mf[c] + mf[c + sg]) * 0.5, ml = (mf[c] + mf[c - sg]) * 0.5;
+            const SH* mf = ws + (long long)fMOM(f) * N;
+            const double mc = mf[c];
+            const double mr = (mc + (double)mf[c + sg]) * 0.5, ml = (mc + (double)mf[c - sg]) * 0.5;
             const double dmv = (rr * nur * gr - rl * nul * gl) * (rdx[dim] * 0.5);
             const double dev = (mr * nur * gr - ml * nul * gl) * (rdx[dim] * 0.5);
             acc[fMOM(f)] += dmv;
@@ -434,7 +442,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (G.hyper && (T & SMAUG_T_HB)) {
 #pragma unroll
     for (int f = 0; f < ND; f++) {
-      stage_q<ND>(G, w + (long long)fB(ND, f) * N, qb, o);
+      stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o);
       __syncthreads();
 #pragma unroll
       for (int dim = 0; dim < ND; dim++) {
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
         const int rs[3] = {1, rx, rx * ry};
         const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
-        const double* bd = w + (long long)fB(ND, dim) * N;
+        const SH* bd = ws + (long long)fB(ND, dim) * N;
         for (int l = tid; l < rx * ry * rz; l += 256) {
           const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
           int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
@@ -454,10 +462,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const long long gr_ = gix(G, xr[0], xr[1], xr[2]);
             const double dxr = A.delx[(long long)dim * N + gr_], rdxr = A.rdelx[(long long)dim * N + gr_];
             nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
-            const double t2 = z1(G, zTMP1, dim, xr[dim]) ? 0.0 : (qb[qi + qs[dim]] - qb[qi - qs[dim]]) * rdxr * 0.5;
+            const double t2 = z1(G, zTMP1, dim, xr[dim]) ? 0.0 : ((double)qb[qi + qs[dim]] - (double)qb[qi - qs[dim]]) * rdxr * 0.5;
             const double t3 = t2 * (nl + nr) * 0.5;
             g3 = t3;
-            g5 = t3 * bd[gr_];
+            g5 = t3 * (double)bd[gr_];
           }
           NR[l] = nr;
           NL[l] = nl;
@@ -474,10 +482,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           acc[fE(ND)] += -1.0 * o5;
           const int s = qs[dim];
           const bool z = z1(G, zTMP1, dim, x[dim]);
-          const double q0 = qb[qc];
-          const double gr = z ? 0.0 : (qb[qc + s] - q0) * rdx[dim];
-          const double gl = z ? 0.0 : (q0 - qb[qc - s]) * rdx[dim];
-          const double br = (q0 + qb[qc + s]) * 0.5, bl = (q0 + qb[qc - s]) * 0.5;
+          const double q0 = (double)qb[qc];
+          const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[dim];
+          const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[dim];
+          const double br = (q0 + (double)qb[qc + s]) * 0.5, bl = (q0 + (double)qb[qc - s]) * 0.5;
           acc[fB(ND, f)] += (gr * nur - gl * nul) * rdx[dim];
           acc[fE(ND)] += (br * gr * nur - bl * gl * nul) * rdx[dim];
         }
@@ -547,16 +555,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
 }
 
+template <int ND, typename SH>
+static void launch_tiled_t(const Geo& G, hipStream_t st, const RhsArgs& A) {
+  using L = Lay<ND>;
+  int nb = ((G.n[0] - 4 + L::TX - 1) / L::TX) * ((G.n[1] - 4 + L::TY - 1) / L::TY);
+  if (ND == 3) nb *= (G.n[2] - 4 + L::TZ - 1) / L::TZ;
+  hipLaunchKernelGGL((k_rhs_tiled<ND, SH>), dim3(nb), dim3(256), 0, st, G, A);
+}
+
 void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A) {
   if (G.nd == 3) {
-    using L = Lay<3>;
-    const int nb = ((G.n[0] - 4 + L::TX - 1) / L::TX) * ((G.n[1] - 4 + L::TY - 1) / L::TY) *
-                   ((G.n[2] - 4 + L::TZ - 1) / L::TZ);
-    hipLaunchKernelGGL(k_rhs_tiled<3>, dim3(nb), dim3(256), 0, st, G, A);
+    if (G.mixed) launch_tiled_t<3, float>(G, st, A);
+    else launch_tiled_t<3, double>(G, st, A);
   } else {
-    using L = Lay<2>;
-    const int nb = ((G.n[0] - 4 + L::TX - 1) / L::TX) * ((G.n[1] - 4 + L::TY - 1) / L::TY);
-    hipLaunchKernelGGL(k_rhs_tiled<2>, dim3(nb), dim3(256), 0, st, G, A);
+    if (G.mixed) launch_tiled_t<2, float>(G, st, A);
+    else launch_tiled_t<2, double>(G, st, A);
   }
 }
 

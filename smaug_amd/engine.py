@@ -15,7 +15,7 @@ import numpy as np
 from .params import RunParams, nvar
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsmaughip.so")
+LIB_PATH = os.environ.get("SMAUG_HIP_LIB", os.path.join(_HERE, "libsmaughip.so"))
 _lib = None
 
 EXPORTS = [

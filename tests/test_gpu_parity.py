@@ -107,3 +107,22 @@ def test_cfl_matches_oracle():
     c2, dt2 = orc.cfl()
     assert np.array_equal(c1, c2), (c1, c2)
     assert dt1 == dt2
+
+
+def test_strat3d_many_interior_tiles():
+    """A grid large enough that most workgroups take the interior fast
+    paths (no halo substitutions, no boundary zeroing) and the tile order
+    spreads bricks of tiles over the XCDs: 32 x 32 x 64 interior, RK4."""
+    rp, w, pos = strat_case(ni=(32, 32, 64), ndim=3, rkon=1)
+    a, b, ca, cb = run_pair(rp, w, pos, 2)
+    err = compare(rp, a, b)
+    print(f"strat3d 32x32x64 RK4 x2 rel_l2={err}", flush=True)
+    assert np.all(err <= TOL), err
+
+
+def test_strat2d_many_interior_tiles():
+    rp, w, pos = strat_case(ni=(96, 64), ndim=2, rkon=1)
+    a, b, ca, cb = run_pair(rp, w, pos, 3)
+    err = compare(rp, a, b)
+    print(f"strat2d 96x64 RK4 x3 rel_l2={err}", flush=True)
+    assert np.all(err <= TOL), err
