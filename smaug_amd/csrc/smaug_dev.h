@@ -140,6 +140,8 @@ struct RhsArgs {
   const void* prim;     // k_prim output of win: double (FP64) or float (mixed)
   const double* delx;   // nd*N
   const double* rdelx;  // nd*N, 1/delx
+  const double* dx1;    // separable grid: delx along d at coordinate x is
+  const double* rdx1;   // dx1[off(d) + x], off = 0, n0, n0+n1 (else null)
   const double* pos;    // nd*N
   const double* base;   // stage base (w^n), read at the cell only
   const double* r1;     // RK combination inputs (final stage only, else null)

@@ -61,6 +61,8 @@ struct smaug_ctx {
   float* shadow = nullptr;  // mixed: FP32 copy of the stage input for the stencils
   double* delx = nullptr;
   double* rdelx = nullptr;
+  double* dx1 = nullptr;   // separable grid: per-coordinate spacing (n0+n1+n2)
+  double* rdx1 = nullptr;
   double* pos = nullptr;
   unsigned long long* cmax = nullptr;  // 3 slots of 4
   double* dts = nullptr;               // dt, t
@@ -195,6 +197,24 @@ static void make_geo(const smaug_params* p, Geo& G) {
   G.mixed = p->precision == SMAUG_PREC_MIXED;
 }
 
+// delx along every d depends only on the d coordinate: returns the spacing
+// per coordinate (n0 values of dim 0, then n1 of dim 1, ...) in dx1
+static bool separable_spacing(int nd, const int* n, const std::vector<double>& delx, std::vector<double>& dx1) {
+  const long long n0 = n[0], n1 = n[1], n2 = nd == 3 ? n[2] : 1, N = n0 * n1 * n2;
+  dx1.clear();
+  for (int d = 0; d < nd; d++) {
+    const long long st = d == 0 ? 1 : (d == 1 ? n0 : n0 * n1);
+    const int nn = n[d];
+    const double* a = delx.data() + (size_t)d * N;
+    for (int x = 0; x < nn; x++) dx1.push_back(a[x * st]);
+    for (long long c = 0; c < N; c++) {
+      const int x = (int)((c / st) % nn);
+      if (a[c] != dx1[dx1.size() - nn + x]) return false;
+    }
+  }
+  return true;
+}
+
 static void free_ctx(smaug_t* s) {
   if (!s) return;
   for (auto& u : s->U)
@@ -204,6 +224,8 @@ static void free_ctx(smaug_t* s) {
   hipFree(s->shadow);
   hipFree(s->delx);
   hipFree(s->rdelx);
+  hipFree(s->dx1);
+  hipFree(s->rdx1);
   hipFree(s->pos);
   hipFree(s->cmax);
   hipFree(s->dts);
@@ -251,6 +273,22 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   initgrid_host(s->nd, p->n, pos, s->h_pos, s->h_delx);
   std::vector<double> h_rdelx(s->h_delx.size());
   for (size_t q = 0; q < h_rdelx.size(); q++) h_rdelx[q] = 1.0 / s->h_delx[q];
+  // tensor-product grid (delx along d depends on the d coordinate only, the
+  // case of every grid built from per-axis positions): the kernels read
+  // one spacing per coordinate instead of the nd*N arrays
+  std::vector<double> h_dx1, h_rdx1;
+  if (separable_spacing(s->nd, p->n, s->h_delx, h_dx1)) {
+    h_rdx1.resize(h_dx1.size());
+    for (size_t q = 0; q < h_dx1.size(); q++) h_rdx1[q] = 1.0 / h_dx1[q];
+    ok &= hipMalloc(&s->dx1, sizeof(double) * h_dx1.size()) == hipSuccess;
+    ok &= hipMalloc(&s->rdx1, sizeof(double) * h_dx1.size()) == hipSuccess;
+    ok = ok && hipMemcpy(s->dx1, h_dx1.data(), sizeof(double) * h_dx1.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(s->rdx1, h_rdx1.data(), sizeof(double) * h_dx1.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+      free_ctx(s);
+      return fail(SMAUG_E_NOMEM, "device allocation failed");
+    }
+  }
   const double dts0[2] = {p->moddton == 1 ? 1.0e-8 : p->dt, 0.0};  // src/iosmaug.c:105,333-336
   const unsigned long long z12[12] = {0};
   const int z = 0;
@@ -287,6 +325,8 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
   A.prim = s->prim;
   A.delx = s->delx;
   A.rdelx = s->rdelx;
+  A.dx1 = s->dx1;
+  A.rdx1 = s->rdx1;
   A.pos = s->pos;
   A.base = base;
   A.r1 = A.r2 = A.r3 = nullptr;

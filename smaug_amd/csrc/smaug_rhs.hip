@@ -238,8 +238,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   double dx[ND], rdx[ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    dx[d] = valid ? A.delx[(long long)d * N + c] : 1.0;
-    rdx[d] = valid ? A.rdelx[(long long)d * N + c] : 1.0;
+    if (A.dx1) {  // tensor-product grid: one spacing per coordinate
+      const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
+      dx[d] = valid ? A.dx1[od + x[d]] : 1.0;
+      rdx[d] = valid ? A.rdx1[od + x[d]] : 1.0;
+    } else {
+      dx[d] = valid ? A.delx[(long long)d * N + c] : 1.0;
+      rdx[d] = valid ? A.rdelx[(long long)d * N + c] : 1.0;
+    }
   }
 
   // ------------------------------------------------------------ fluxes
@@ -382,7 +388,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
               if (interior_c<ND>(G, xr)) {
                 const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
                 const long long gq = gix(G, xr[0], xr[1], xr[2]);
-                const double dxr = A.delx[(long long)dim * N + gq], rdxr = A.rdelx[(long long)dim * N + gq];
+                const int od = dim == 0 ? 0 : (dim == 1 ? G.n[0] : G.n[0] + G.n[1]);
+                const double dxr = A.dx1 ? A.dx1[od + xr[dim]] : A.delx[(long long)dim * N + gq];
+                const double rdxr = A.dx1 ? A.rdx1[od + xr[dim]] : A.rdelx[(long long)dim * N + gq];
                 nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
                 const double t5 =
                     z1(G, zTMP4, dim, xr[dim]) ? 0.0 : ((double)qb[qi + qs[dim]] - (double)qb[qi - qs[dim]]) * rdxr * 0.5;
@@ -460,7 +468,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (interior_c<ND>(G, xr)) {
             const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
             const long long gr_ = gix(G, xr[0], xr[1], xr[2]);
-            const double dxr = A.delx[(long long)dim * N + gr_], rdxr = A.rdelx[(long long)dim * N + gr_];
+            const int od = dim == 0 ? 0 : (dim == 1 ? G.n[0] : G.n[0] + G.n[1]);
+            const double dxr = A.dx1 ? A.dx1[od + xr[dim]] : A.delx[(long long)dim * N + gr_];
+            const double rdxr = A.dx1 ? A.rdx1[od + xr[dim]] : A.rdelx[(long long)dim * N + gr_];
             nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
             const double t2 = z1(G, zTMP1, dim, xr[dim]) ? 0.0 : ((double)qb[qi + qs[dim]] - (double)qb[qi - qs[dim]]) * rdxr * 0.5;
             const double t3 = t2 * (nl + nr) * 0.5;

@@ -126,3 +126,21 @@ def test_strat2d_many_interior_tiles():
     err = compare(rp, a, b)
     print(f"strat2d 96x64 RK4 x3 rel_l2={err}", flush=True)
     assert np.all(err <= TOL), err
+
+
+def test_nonseparable_grid():
+    """Positions that are not a tensor product (x stretched with y), so the
+    kernels read the full per-cell spacing arrays instead of per-axis ones."""
+    rp, w, pos = ot_case(ni=20, rkon=1)
+    rp.moddton = 0
+    rp.dt = 1.0e-3
+    pos = pos.copy()
+    pos[0] = pos[0] * (1.0 + 0.02 * np.sin(2 * np.pi * pos[1]))
+    a, b, ca, cb = run_pair(rp, w, pos, 3)
+    err = compare(rp, a, b)
+    print(f"OT non-separable grid RK4 x3 rel_l2={err}", flush=True)
+    assert np.all(err <= TOL), err
+    eng = Engine(rp, w, pos)
+    _, d1 = eng.grid()
+    eng.close()
+    assert not np.allclose(d1[0].reshape(rp.n[1], rp.n[0])[3], d1[0].reshape(rp.n[1], rp.n[0])[7])
