@@ -203,7 +203,7 @@ __device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const dou
 }
 
 template <int ND, typename SH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rhs_tiled(Geo G, RhsArgs A) {
+__global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
   constexpr int NP = L::NP;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
@@ -290,9 +290,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // ------------------------------------------------------------ hyperdiffusion + energy terms
   SH* qb = reinterpret_cast<SH*>(sm);                                        // q box
   double* NR = sm + (L::QN * sizeof(SH) + sizeof(double) - 1) / sizeof(double);  // region arrays
-  double* NL = NR + L::RN;
-  double* GA = NL + L::RN;
-  double* GB = GA + L::RN;
+  double* RG[4] = {NR, NR + L::RN, NR + 2 * L::RN, NR + 3 * L::RN};  // GA/GB of the two cross directions
   // tile-cell index in the q box and grid strides
   const int qc = (lc[0] + 3) + L::QX * ((lc[1] + 3) + L::QY * (ND == 3 ? lc[2] + 3 : 0));
   const int qs[3] = {1, L::QX, L::QX * L::QY};
@@ -371,136 +369,204 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         acc[fE(ND)] += t;
       }
       if (want_mom) {
+        // one region pass per f for both cross directions: the tile
+        // extended by one layer along f, products rho_t*t6 and t6*m_d for
+        // each d != f; the own cell's nu in every direction stays in
+        // registers (src/hyperdifmomsource1.cu, src/hyperdifmomsourcene1.cu)
+        const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
+        const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1];
+        const int sf = f == 0 ? 1 : (f == 1 ? rx : rx * ry);
+        const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+        const double Kc = G.chyp[fMOM(f)];
+        double nur[ND], nul[ND];
 #pragma unroll
-        for (int dim = 0; dim < ND; dim++) {
-          const double K = cm[dim] * G.chyp[fMOM(f)];
-          const bool ne = f != dim;
-          double nur = 0, nul = 0;
-          const int ex[3] = {ne && f == 0 ? 1 : 0, ne && f == 1 ? 1 : 0, ne && f == 2 ? 1 : 0};
-          const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
-          const int rs[3] = {1, rx, rx * ry};
-          const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
-          if (ne) {
-            for (int l = tid; l < rx * ry * rz; l += 256) {
-              const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
-              int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
-              double nr = 0, nl = 0, g7 = 0, g8 = 0;
-              if (interior_c<ND>(G, xr)) {
-                const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
-                const long long gq = gix(G, xr[0], xr[1], xr[2]);
-                const int od = dim == 0 ? 0 : (dim == 1 ? G.n[0] : G.n[0] + G.n[1]);
-                const double dxr = A.dx1 ? A.dx1[od + xr[dim]] : A.delx[(long long)dim * N + gq];
-                const double rdxr = A.dx1 ? A.rdx1[od + xr[dim]] : A.rdelx[(long long)dim * N + gq];
-                nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
-                const double t5 =
-                    z1(G, zTMP4, dim, xr[dim]) ? 0.0 : ((double)qb[qi + qs[dim]] - (double)qb[qi - qs[dim]]) * rdxr * 0.5;
-                const double t6 = t5 * (nr + nl) * 0.25;
-                g7 = ((double)ws[gq] + bg[(long long)bRHOB() * N + gq]) * t6;
-                g8 = t6 * (double)ws[(long long)fMOM(dim) * N + gq];
-              }
-              NR[l] = nr;
-              NL[l] = nl;
-              GA[l] = g7;
-              GB[l] = g8;
-            }
-            __syncthreads();
+        for (int d = 0; d < ND; d++) {
+          nur[d] = nul[d] = 0.0;
+          if (valid) nu_win(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
+          if (d != f) {
+            const int k = d < f ? d : d - 1;
+            double g7 = 0, g8 = 0;
             if (valid) {
-              nur = NR[ri];
-              nul = NL[ri];
+              const double t5 =
+                  z1(G, zTMP4, d, x[d]) ? 0.0 : ((double)qb[qc + qs[d]] - (double)qb[qc - qs[d]]) * rdx[d] * 0.5;
+              const double t6 = t5 * (nur[d] + nul[d]) * 0.25;
+              g7 = rt_c * t6;
+              g8 = t6 * (double)ws[(long long)fMOM(d) * N + c];
             }
-          } else if (valid) {
-            nu_win(qb, qc, qs[dim], dx[dim] * K, nur, nul);
+            RG[2 * k][ri] = g7;
+            RG[2 * k + 1][ri] = g8;
           }
-          if (valid) {
-            const int s = qs[dim];
-            const long long sg = gs[dim];
-            const bool z = z1(G, zTMP4, dim, x[dim]);
+        }
+        {  // the two layers just outside the tile along f
+          const int pd = f == 0 ? 1 : 0, qd = f == 2 ? 1 : 2;
+          const int Tp = pd == 0 ? TX : TY, Tq = ND == 3 ? (qd == 1 ? TY : TZ) : 1;
+          const int Fc = Tp * Tq;
+          if (tid < 2 * Fc) {
+            const int side = tid >= Fc ? 1 : 0, r = tid - side * Fc;
+            int bc[3] = {0, 0, 0};
+            bc[f] = side ? (f == 0 ? TX : (f == 1 ? TY : TZ)) + 1 : 0;
+            bc[pd] = r % Tp;
+            if (ND == 3) bc[qd] = r / Tp;
+            const int l = bc[0] + rx * (bc[1] + ry * bc[2]);
+            const int rel[3] = {bc[0] - ex[0], bc[1] - ex[1], ND == 3 ? bc[2] - ex[2] : 0};
+            int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
+            const bool in = interior_c<ND>(G, xr);
+            const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
+            const double rt = in ? (double)ws[gq] + bg[(long long)bRHOB() * N + gq] : 0.0;
+            const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+              if (d == f) continue;
+              const int k = d < f ? d : d - 1;
+              double g7 = 0, g8 = 0;
+              if (in) {
+                const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
+                const double dxr = A.dx1 ? A.dx1[od + xr[d]] : A.delx[(long long)d * N + gq];
+                const double rdxr = A.dx1 ? A.rdx1[od + xr[d]] : A.rdelx[(long long)d * N + gq];
+                double nr, nl;
+                nu_win(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
+                const double t5 =
+                    z1(G, zTMP4, d, xr[d]) ? 0.0 : ((double)qb[qi + qs[d]] - (double)qb[qi - qs[d]]) * rdxr * 0.5;
+                const double t6 = t5 * (nr + nl) * 0.25;
+                g7 = rt * t6;
+                g8 = t6 * (double)ws[(long long)fMOM(d) * N + gq];
+              }
+              RG[2 * k][l] = g7;
+              RG[2 * k + 1][l] = g8;
+            }
+          }
+        }
+        __syncthreads();
+        if (valid) {
+          const double* rb = bg + (long long)bRHOB() * N;
+          const SH* mf = ws + (long long)fMOM(f) * N;
+          const double mc = mf[c];
+#pragma unroll
+          for (int d = 0; d < ND; d++) {
+            const int s = qs[d];
+            const long long sg = gs[d];
+            const bool z = z1(G, zTMP4, d, x[d]);
             const double q0 = (double)qb[qc];
-            const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[dim];
-            const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[dim];
-            const double* rb = bg + (long long)bRHOB() * N;
+            const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[d];
+            const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[d];
             const double rtp = (double)ws[c + sg] + rb[c + sg], rtm = (double)ws[c - sg] + rb[c - sg];
             const double rr = (rt_c + rtp) * 0.5, rl = (rt_c + rtm) * 0.5;
-            const SH* mf = ws + (long long)fMOM(f) * N;
-            const double mc = mf[c];
             const double mr = (mc + (double)mf[c + sg]) * 0.5, ml = (mc + (double)mf[c - sg]) * 0.5;
-            const double dmv = (rr * nur * gr - rl * nul * gl) * (rdx[dim] * 0.5);
-            const double dev = (mr * nur * gr - ml * nul * gl) * (rdx[dim] * 0.5);
+            const double dmv = (rr * nur[d] * gr - rl * nul[d] * gl) * (rdx[d] * 0.5);
+            const double dev = (mr * nur[d] * gr - ml * nul[d] * gl) * (rdx[d] * 0.5);
             acc[fMOM(f)] += dmv;
             acc[fE(ND)] += dev;
-            if (!ne) {
+            if (d == f) {
               acc[fMOM(f)] += dmv;
               acc[fE(ND)] += dev;
             } else {
-              const int sf = rs[f];
-              const double o7 = z1(G, zTMP7, f, x[f]) ? 0.0 : (GA[ri + sf] - GA[ri - sf]) * (rdx[f] * 0.5);
-              const double o8 = z1(G, zTMP8, f, x[f]) ? 0.0 : (GB[ri + sf] - GB[ri - sf]) * (rdx[f] * 0.5);
-              acc[fMOM(dim)] += o7;
+              const int k = d < f ? d : d - 1;
+              const double* GAk = RG[2 * k];
+              const double* GBk = RG[2 * k + 1];
+              const double o7 = z1(G, zTMP7, f, x[f]) ? 0.0 : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * 0.5);
+              const double o8 = z1(G, zTMP8, f, x[f]) ? 0.0 : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * 0.5);
+              acc[fMOM(d)] += o7;
               acc[fE(ND)] += o8;
             }
           }
-          if (ne) __syncthreads();
         }
       }
       __syncthreads();
     }
   }
 
-  // magnetic phases (q = b_f): off-diagonal (sb=-1) and diagonal (sb=+1)
+  // magnetic phases (q = b_f): curl-form cross terms (sb = -1,
+  // src/hyperdifbsourcene1.cu) and diagonal terms (sb = +1,
+  // src/hyperdifbsource1.cu), one region pass per f for both d != f
   if (G.hyper && (T & SMAUG_T_HB)) {
 #pragma unroll
     for (int f = 0; f < ND; f++) {
       stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o);
       __syncthreads();
+      const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
+      const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1];
+      const int sf = f == 0 ? 1 : (f == 1 ? rx : rx * ry);
+      const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+      const double Kc = G.chyp[fB(ND, f)];
+      double nur[ND], nul[ND];
 #pragma unroll
-      for (int dim = 0; dim < ND; dim++) {
-        if (dim == f) continue;
-        const double K = cm[dim] * G.chyp[fB(ND, f)];
-        const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
-        const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
-        const int rs[3] = {1, rx, rx * ry};
-        const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
-        const SH* bd = ws + (long long)fB(ND, dim) * N;
-        for (int l = tid; l < rx * ry * rz; l += 256) {
-          const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
-          int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
-          double nr = 0, nl = 0, g3 = 0, g5 = 0;
-          if (interior_c<ND>(G, xr)) {
-            const int qi = (a - ex[0] + 3) + L::QX * ((b - ex[1] + 3) + L::QY * (ND == 3 ? cz - ex[2] + 3 : 0));
-            const long long gr_ = gix(G, xr[0], xr[1], xr[2]);
-            const int od = dim == 0 ? 0 : (dim == 1 ? G.n[0] : G.n[0] + G.n[1]);
-            const double dxr = A.dx1 ? A.dx1[od + xr[dim]] : A.delx[(long long)dim * N + gr_];
-            const double rdxr = A.dx1 ? A.rdx1[od + xr[dim]] : A.rdelx[(long long)dim * N + gr_];
-            nu_win(qb, qi, qs[dim], dxr * K, nr, nl);
-            const double t2 = z1(G, zTMP1, dim, xr[dim]) ? 0.0 : ((double)qb[qi + qs[dim]] - (double)qb[qi - qs[dim]]) * rdxr * 0.5;
-            const double t3 = t2 * (nl + nr) * 0.5;
-            g3 = t3;
-            g5 = t3 * (double)bd[gr_];
-          }
-          NR[l] = nr;
-          NL[l] = nl;
-          GA[l] = g3;
-          GB[l] = g5;
-        }
-        __syncthreads();
+      for (int d = 0; d < ND; d++) {
+        nur[d] = nul[d] = 0.0;
+        if (d == f) continue;
+        const int k = d < f ? d : d - 1;
+        double g3 = 0, g5 = 0;
         if (valid) {
-          const double nur = NR[ri], nul = NL[ri];
-          const int sf = rs[f];
-          const double o3 = z1(G, zTMP3, f, x[f]) ? 0.0 : (GA[ri + sf] - GA[ri - sf]) * (rdx[f] * 0.5);
-          const double o5 = z1(G, zTMP5, f, x[f]) ? 0.0 : (GB[ri + sf] - GB[ri - sf]) * (rdx[f] * 0.5);
-          acc[fB(ND, dim)] += -1.0 * o3;
-          acc[fE(ND)] += -1.0 * o5;
-          const int s = qs[dim];
-          const bool z = z1(G, zTMP1, dim, x[dim]);
-          const double q0 = (double)qb[qc];
-          const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[dim];
-          const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[dim];
-          const double br = (q0 + (double)qb[qc + s]) * 0.5, bl = (q0 + (double)qb[qc - s]) * 0.5;
-          acc[fB(ND, f)] += (gr * nur - gl * nul) * rdx[dim];
-          acc[fE(ND)] += (br * gr * nur - bl * gl * nul) * rdx[dim];
+          nu_win(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
+          const double t2 =
+              z1(G, zTMP1, d, x[d]) ? 0.0 : ((double)qb[qc + qs[d]] - (double)qb[qc - qs[d]]) * rdx[d] * 0.5;
+          const double t3 = t2 * (nul[d] + nur[d]) * 0.5;
+          g3 = t3;
+          g5 = t3 * (double)ws[(long long)fB(ND, d) * N + c];
         }
-        __syncthreads();
+        RG[2 * k][ri] = g3;
+        RG[2 * k + 1][ri] = g5;
       }
+      {  // the two layers just outside the tile along f
+        const int pd = f == 0 ? 1 : 0, qd = f == 2 ? 1 : 2;
+        const int Tp = pd == 0 ? TX : TY, Tq = ND == 3 ? (qd == 1 ? TY : TZ) : 1;
+        const int Fc = Tp * Tq;
+        if (tid < 2 * Fc) {
+          const int side = tid >= Fc ? 1 : 0, r = tid - side * Fc;
+          int bc[3] = {0, 0, 0};
+          bc[f] = side ? (f == 0 ? TX : (f == 1 ? TY : TZ)) + 1 : 0;
+          bc[pd] = r % Tp;
+          if (ND == 3) bc[qd] = r / Tp;
+          const int l = bc[0] + rx * (bc[1] + ry * bc[2]);
+          const int rel[3] = {bc[0] - ex[0], bc[1] - ex[1], ND == 3 ? bc[2] - ex[2] : 0};
+          int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
+          const bool in = interior_c<ND>(G, xr);
+          const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
+          const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
+#pragma unroll
+          for (int d = 0; d < ND; d++) {
+            if (d == f) continue;
+            const int k = d < f ? d : d - 1;
+            double g3 = 0, g5 = 0;
+            if (in) {
+              const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
+              const double dxr = A.dx1 ? A.dx1[od + xr[d]] : A.delx[(long long)d * N + gq];
+              const double rdxr = A.dx1 ? A.rdx1[od + xr[d]] : A.rdelx[(long long)d * N + gq];
+              double nr, nl;
+              nu_win(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
+              const double t2 =
+                  z1(G, zTMP1, d, xr[d]) ? 0.0 : ((double)qb[qi + qs[d]] - (double)qb[qi - qs[d]]) * rdxr * 0.5;
+              const double t3 = t2 * (nl + nr) * 0.5;
+              g3 = t3;
+              g5 = t3 * (double)ws[(long long)fB(ND, d) * N + gq];
+            }
+            RG[2 * k][l] = g3;
+            RG[2 * k + 1][l] = g5;
+          }
+        }
+      }
+      __syncthreads();
+      if (valid) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+          if (d == f) continue;
+          const int k = d < f ? d : d - 1;
+          const double* GAk = RG[2 * k];
+          const double* GBk = RG[2 * k + 1];
+          const double o3 = z1(G, zTMP3, f, x[f]) ? 0.0 : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * 0.5);
+          const double o5 = z1(G, zTMP5, f, x[f]) ? 0.0 : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * 0.5);
+          acc[fB(ND, d)] += -1.0 * o3;
+          acc[fE(ND)] += -1.0 * o5;
+          const int s = qs[d];
+          const bool z = z1(G, zTMP1, d, x[d]);
+          const double q0 = (double)qb[qc];
+          const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[d];
+          const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[d];
+          const double br = (q0 + (double)qb[qc + s]) * 0.5, bl = (q0 + (double)qb[qc - s]) * 0.5;
+          acc[fB(ND, f)] += (gr * nur[d] - gl * nul[d]) * rdx[d];
+          acc[fE(ND)] += (br * gr * nur[d] - bl * gl * nul[d]) * rdx[d];
+        }
+      }
+      __syncthreads();
     }
   }
 
