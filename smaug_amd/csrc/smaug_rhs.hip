@@ -160,18 +160,35 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
 }
 
 // fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,
-// src/centdiff2.cu:158-356), from v = m/rho_t and p_t of k_prim
+// src/centdiff2.cu:158-356)
 template <int ND, typename SH>
-__device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg,
-                               const SH* __restrict__ pr, long long cc, int d, double* F) {
+__device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
+                               int d, double* F) {
+  // v = m/rho_t and p_t are recomputed from the cell's state (4 fewer
+  // loads per flux cell than reading k_prim's copies); p_t follows
+  // computept3, src/dervfields.cu:107-153
   const long long N = G.N;
-  double v[ND], b[ND], bb[ND];
+  double m[ND], b[ND], bb[ND];
 #pragma unroll
   for (int q = 0; q < ND; q++) {
-    v[q] = pr[(long long)pV(q) * N + cc];
+    m[q] = w[(long long)fMOM(q) * N + cc];
     b[q] = w[(long long)fB(ND, q) * N + cc];
     bb[q] = bg[(long long)bBB(q) * N + cc];
   }
+  const double rho = w[cc], rhob = bg[(long long)bRHOB() * N + cc];
+  const double e = w[(long long)fE(ND) * N + cc], eb = bg[(long long)bEB() * N + cc];
+  const double rt = rho + rhob;
+  double v[ND];
+#pragma unroll
+  for (int q = 0; q < ND; q++) v[q] = m[q] / rt;
+  double mm = m[0] * m[0], b2 = b[0] * b[0], bbb = b[0] * bb[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++) {
+    mm = mm + m[q] * m[q];
+    b2 = b2 + b[q] * b[q];
+    bbb = bbb + b[q] * bb[q];
+  }
+  const double pt = (G.gamma - 1.0) * (e - 0.5 * mm / rt) - (G.gamma - 2.0) * (bbb + 0.5 * b2);
   double vd = v[0], bd = b[0], bbd = bb[0];
 #pragma unroll
   for (int q = 1; q < ND; q++)
@@ -180,14 +197,11 @@ __device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const dou
       bd = b[q];
       bbd = bb[q];
     }
-  const double rho = w[cc], rhob = bg[(long long)bRHOB() * N + cc];
-  const double pt = pr[(long long)pPT(ND) * N + cc];
   F[fRHO()] = vd * rho + rhob * vd;
 #pragma unroll
   for (int f = 0; f < ND; f++) {
-    const double mf = w[(long long)fMOM(f) * N + cc];
     const double mag = -(b[f] * bbd + bb[f] * bd) - b[f] * bd;
-    F[fMOM(f)] = vd * mf + (f == d ? pt + mag : mag);
+    F[fMOM(f)] = vd * m[f] + (f == d ? pt + mag : mag);
   }
   double bbv = bb[0] * v[0], bv = b[0] * v[0];
 #pragma unroll
@@ -195,7 +209,6 @@ __device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const dou
     bbv = bbv + bb[q] * v[q];
     bv = bv + b[q] * v[q];
   }
-  const double e = w[(long long)fE(ND) * N + cc], eb = bg[(long long)bEB() * N + cc];
   F[fE(ND)] = vd * e + ((vd * pt - bd * bbv - bbd * bv) + vd * eb - bd * bv);
 #pragma unroll
   for (int f = 0; f < ND; f++)
@@ -260,7 +273,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
         double F[NP];
         if (in_grid<ND>(G, xr)) {
-          fluxes2<ND, SH>(G, ws, bg, pr, gix(G, xr[0], xr[1], xr[2]), d, F);
+          fluxes2<ND, SH>(G, ws, bg, gix(G, xr[0], xr[1], xr[2]), d, F);
         } else {
 #pragma unroll
           for (int f = 0; f < NP; f++) F[f] = 0.0;
