@@ -165,6 +165,53 @@ int smaug_out_write(const char* path, const char* name, int ndim, const int* n, 
                     const double* pos, int it, double t, double gamma, double eta, const double* g);
 int smaug_write_out(smaug_t* s, const char* prefix, const char* name);
 
+/* ---- driver plug-in, streams, checkpoints --------------------------------
+ *   smaug_set_driver_map  <- the amplitude of the vertical-velocity driver
+ *                            (models/usersource.cu.fluxtube1:3-90, installed
+ *                            as addsourceterms2, src/centdiff2.cu:790-872),
+ *                            given per (j,k) column: n1*n2 values, column
+ *                            (j,k) at k*n1+j, ghost columns included.  It
+ *                            replaces A exp(-dx^2/sxy^2 [- dy^2/sxy^2]):
+ *                            v_z = amap(j,k) exp(-dz^2/sz^2) sin(2 pi qt/P).
+ *                            amap NULL restores the scalar form; on_device=1
+ *                            when amap is a device pointer.
+ *   smaug_set_stream      -- queue all further work on a caller's HIP stream
+ *                            (NULL: the context's own stream).
+ *   smaug_set_clock       -- step counter and nominal clock p->qt (to restart
+ *                            from a checkpoint).
+ *   smaug_save_pert / smaug_load_pert -- device-to-device copy of the current
+ *                            evolved fields (np*N doubles, np = 2+2*ndim) to /
+ *                            from a caller's device buffer; load re-fills the
+ *                            ghost cells. */
+int smaug_set_driver_map(smaug_t* s, const double* amap, int on_device);
+int smaug_set_stream(smaug_t* s, void* hip_stream);
+int smaug_set_clock(smaug_t* s, int it, double qt);
+int smaug_save_pert(const smaug_t* s, double* dst_device);
+int smaug_load_pert(smaug_t* s, const double* src_device);
+
+/* ---- adjoint (reverse-mode derivative) of the step ------------------------
+ * No counterpart in the reference: the tangent/adjoint step entry points of
+ * the differentiable binding.  All buffers are device pointers.
+ *   smaug_tape_doubles  -- size of one step's tape: the stage inputs of the
+ *                          step (4*np*N doubles for rkon=1, np*N for rkon=0)
+ *                          and 8 doubles of metadata (dt, qt, driver factor).
+ *   smaug_step_taped    -- smaug_step(s, 1), also writing the step's tape.
+ *   smaug_adjoint_step  -- given the tape of a step, the state after it
+ *                          (wnext, np*N) and in lam (np*N) the cotangent of
+ *                          that state, overwrites lam with the cotangent of the
+ *                          state before the step and adds d(loss)/d(amap)
+ *                          (per column, n1*n2 values; NULL to skip) to gmap.
+ *                          The hyperdiffusion coefficients (windowed maxima)
+ *                          and wave-speed maxima are held at their forward
+ *                          values, and dt is the taped one.  Cotangents live
+ *                          on interior cells (ghost entries of lam are zero on
+ *                          return; a ghost cotangent on entry is ignored, so
+ *                          fold it onto its image first). */
+long long smaug_tape_doubles(const smaug_t* s);
+int smaug_step_taped(smaug_t* s, double* tape_device);
+int smaug_adjoint_step(smaug_t* s, const double* tape_device, const double* wnext_device, double* lam_device,
+                       double* gmap_device);
+
 /* Kernel timing hooks for the benchmark: launches `reps` RK stages of the
  * configured step on the current state and reports the mean device time of
  * each kernel family measured with HIP events on the context's stream. */

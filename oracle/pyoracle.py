@@ -40,6 +40,10 @@ def lib():
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_probe_grad.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_probe_visc.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.orc_set_driver_map.argtypes = [vp, dp]
+        L.orc_restart.argtypes = [vp, dp, ctypes.c_int, ctypes.c_double]
+        L.orc_nu_mode.argtypes = [vp, ctypes.c_int]
+        L.orc_nu_mode.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -118,6 +122,30 @@ class Oracle:
         if rc != 0:
             raise ValueError("orc_probe_visc rejected its arguments")
         return nur, nul, cm.value
+
+    def set_driver_map(self, amap):
+        """Driver amplitude per (j, k) column (n1*n2 values), None clears."""
+        if amap is None:
+            lib().orc_set_driver_map(self._h, None)
+            return
+        a = np.ascontiguousarray(amap, dtype=np.float64).reshape(-1)
+        n = self.rp.n
+        if a.size != n[1] * (n[2] if self.nd == 3 else 1):
+            raise ValueError("driver map needs one value per (j, k) column")
+        lib().orc_set_driver_map(self._h, _dp(a))
+
+    def restart(self, w: np.ndarray, it: int = 0, qt: float = 0.0):
+        """Start again from state w (nvar x N) at step it and clock qt."""
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.nvar, self.N)
+        lib().orc_restart(self._h, _dp(w), int(it), float(qt))
+
+    NU_OFF, NU_RECORD, NU_REPLAY = 0, 1, 2
+
+    def nu_mode(self, mode: int) -> int:
+        """Hyperdiffusion coefficients: 0 computed, 1 computed and recorded,
+        2 replayed from the recording (frozen coefficients). Returns the
+        number of recorded coefficient pairs."""
+        return int(lib().orc_nu_mode(self._h, int(mode)))
 
     def close(self):
         if getattr(self, "_h", None):

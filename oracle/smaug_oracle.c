@@ -59,6 +59,10 @@ struct orc {
   int it;
   double cmax_dim[3];
   int cur;        /* slot holding the current state           */
+  double* amap;   /* driver amplitude per column (n1*n2), or NULL */
+  int nu_mode;    /* 0 compute nu, 1 compute and record, 2 replay */
+  double* nu_tape;
+  long nu_len, nu_cap, nu_pos;
 };
 
 /* ---- enums (include/iotypes.h:211-242), as functions of ndim ---------- */
@@ -652,8 +656,35 @@ static void k_visc4(orc_t* o, int dim, int field, int out) {
   }
 }
 
-/* cuhyperdifvisc1ir + cuhyperdifvisc1r + cuhyperdifvisc1l */
+static void k_visc_compute(orc_t* o, int s, int field, int dim);
+
+/* cuhyperdifvisc1ir + cuhyperdifvisc1r + cuhyperdifvisc1l, with the
+ * record/replay tape of the frozen-coefficient derivative checks: mode 1
+ * appends every (nu_r, nu_l) pair computed, mode 2 returns the recorded
+ * pairs in call order instead of computing them. */
 static void k_visc(orc_t* o, int s, int field, int dim) {
+  const long N = o->N;
+  if (o->nu_mode == 2 && o->nu_pos + 2 * N <= o->nu_len) {
+    memcpy(WD(o, D_HDNUR), o->nu_tape + o->nu_pos, sizeof(double) * N);
+    memcpy(WD(o, D_HDNUL), o->nu_tape + o->nu_pos + N, sizeof(double) * N);
+    o->nu_pos += 2 * N;
+    return;
+  }
+  k_visc_compute(o, s, field, dim);
+  if (o->nu_mode == 1) {
+    if (o->nu_len + 2 * N > o->nu_cap) {
+      long cap = o->nu_cap ? 2 * o->nu_cap : 64 * N;
+      while (cap < o->nu_len + 2 * N) cap *= 2;
+      o->nu_tape = (double*)realloc(o->nu_tape, sizeof(double) * cap);
+      o->nu_cap = cap;
+    }
+    memcpy(o->nu_tape + o->nu_len, WD(o, D_HDNUR), sizeof(double) * N);
+    memcpy(o->nu_tape + o->nu_len + N, WD(o, D_HDNUL), sizeof(double) * N);
+    o->nu_len += 2 * N;
+  }
+}
+
+static void k_visc_compute(orc_t* o, int s, int field, int dim) {
   k_visc1r(o, s, field); /* cuhyperdifvisc1ir */
   k_visc1r(o, s, field); /* cuhyperdifvisc1r */
   k_visc1a(o);
@@ -952,6 +983,9 @@ static void k_source(orc_t* o, int sn) {
       exyz = ex * ey * ez;
     }
     double vz = A * exyz * tdep;
+    /* amplitude map: one amplitude per (j,k) column in place of
+     * A*exp(-dx^2/sxy^2 [- dy^2/sxy^2]) */
+    if (o->amap) vz = o->amap[(long)k * o->n1 + j] * ez * tdep;
     double rho_t = W(o, sn, F_RHO)[c] + W(o, sn, F_RHOB)[c];
     W(o, sn, F_MOM(0))[c] += (dt)*vz * rho_t;
     W(o, sn, F_E)[c] += (dt) * (vz * vz) * rho_t / 2.0;
@@ -1324,6 +1358,40 @@ int orc_probe_visc(orc_t* o, int field, int dim, double* nur, double* nul, doubl
   return 0;
 }
 
+/* restart from state w (nvar*N) at step it and clock qt, as orc_create */
+int orc_restart(orc_t* o, const double* w, int it, double qt) {
+  for (int s = 0; s < o->nslot; s++) memcpy(W(o, s, 0), w, sizeof(double) * o->nvar * o->N);
+  o->dt = o->p.moddton == 1 ? 1.0e-8 : o->p.dt;
+  o->qt = qt;
+  o->it = it;
+  o->t = 0;
+  o->cur = o->p.rkon == 0 ? 1 : 0;
+  k_bound_all(o, o->cur);
+  return 0;
+}
+
+/* driver amplitude map (n1*n2 values, column (j,k) at k*n1+j), NULL clears */
+int orc_set_driver_map(orc_t* o, const double* amap) {
+  const long nc = (long)o->n1 * o->n2;
+  if (!amap) {
+    free(o->amap);
+    o->amap = NULL;
+    return 0;
+  }
+  if (!o->amap) o->amap = (double*)malloc(sizeof(double) * nc);
+  memcpy(o->amap, amap, sizeof(double) * nc);
+  return 0;
+}
+
+/* nu tape: mode 0 off, 1 record (clears the tape), 2 replay from the start.
+ * Returns the number of recorded (nu_r, nu_l) pairs. */
+long orc_nu_mode(orc_t* o, int mode) {
+  if (mode == 1) o->nu_len = 0;
+  o->nu_pos = 0;
+  o->nu_mode = mode;
+  return o->nu_len / (2 * o->N);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
@@ -1340,5 +1408,7 @@ void orc_destroy(orc_t* o) {
   free(o->wt1);
   free(o->wt2);
   free(o->dwn1);
+  free(o->amap);
+  free(o->nu_tape);
   free(o);
 }

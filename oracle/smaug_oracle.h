@@ -16,6 +16,9 @@ int orc_cfl(orc_t* o, double* cmax_per_dim, double* dt);
 int orc_get_clock(orc_t* o, int* it, double* t, double* qt, double* dt);
 int orc_probe_grad(orc_t* o, int kind, const double* a, int bcidx, int dir, double* out);
 int orc_probe_visc(orc_t* o, int field, int dim, double* nur, double* nul, double* cmax);
+int orc_restart(orc_t* o, const double* w, int it, double qt);
+int orc_set_driver_map(orc_t* o, const double* amap);
+long orc_nu_mode(orc_t* o, int mode);
 int orc_num_threads(void);
 void orc_destroy(orc_t* o);
 #ifdef __cplusplus

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsmaughip.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("smaug_kernels.hip", "smaug_rhs.hip", "smaug_host.hip", "smaug_io.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("smaug_kernels.hip", "smaug_rhs.hip", "smaug_adj.hip", "smaug_host.hip", "smaug_io.cpp")]
 ARCH = os.environ.get("SMAUG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

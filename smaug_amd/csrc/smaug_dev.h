@@ -153,7 +153,58 @@ struct RhsArgs {
   const double* dts;                    // device dt
   double hfrac;                         // stage step = hfrac*dt
   double tdep;                          // driver time factor sin(2 pi qt/P)
+  const double* amap;                   // driver amplitude per (j,k) column, or null
 };
+
+// Adjoint of one stage (smaug_adj.hip).  The stage maps its input win to
+//   out = base + h*acc(win) + src(out_rho)   [+ (r1 + 2 r2 + r3 - 4 base)/3]
+// on interior cells, with the ghost cells of out images of interior cells.
+// acc is differentiated with the hyperdiffusion coefficients nu and the
+// wave-speed maxima frozen at their values for win.
+struct AdjMuArgs {
+  const double* lam_out;  // cotangent of out (np*N, interior cells)
+  const double* outr;     // rho of the stage output as written (np*N array, field 0)
+  const double* c0;       // combine inputs (base, r1, r2, r3) of the final RK stage, else null
+  const double* c1;
+  const double* c2;
+  const double* c3;
+  const double* bg;
+  const double* pos;
+  const double* amap;
+  const double* meta;     // tape metadata of the step: [0] dt, [2] driver time factor
+  double hfrac;
+  double cb;              // weight of lam_out in the base cotangent (-4/3 on the final RK stage)
+  int base_set;           // 1: lam_base = ..., 0: lam_base += ...
+  double* mu;             // h * cotangent of acc (np*N, zero off the interior)
+  double* lam_base;       // cotangent of base (interior cells)
+  double* lam_r1;         // final RK stage: cotangents of r1..r3 (set), else null
+  double* lam_r2;
+  double* lam_r3;
+  double* gcell;          // per-cell d(loss)/d(driver amplitude), or null
+  int gcell_set;
+};
+
+struct AdjArgs {
+  const double* win;      // stage input (FP64, ghosts filled)
+  const double* bg;
+  const double* prim;     // FP64 k_prim of win
+  const double* delx;
+  const double* rdelx;
+  const double* dx1;
+  const double* rdx1;
+  const double* mu;
+  const unsigned long long* cmax_bits;
+  double* out;            // J^T mu at every cell (np*N)
+};
+
+void launch_adj_mu(const Geo& G, hipStream_t st, const AdjMuArgs& A);
+void launch_adj_rhs(const Geo& G, hipStream_t st, const AdjArgs& A);
+// lam += fold(T): T plus T at every ghost image, on interior cells
+void launch_adj_fold(const Geo& G, hipStream_t st, const double* T, double* lam);
+// gmap[col] += sum over the column of gcell
+void launch_colsum(const Geo& G, hipStream_t st, const double* gcell, double* gmap);
+// meta[0] = dts[0], meta[1] = qt, meta[2] = tdep
+void launch_meta(hipStream_t st, double* meta, const double* dts, double qt, double tdep);
 
 // k_prim: prim (double, or float when mixed) and, when mixed, the FP32
 // shadow of the evolved fields of w

@@ -52,7 +52,8 @@ int io_fail(int code, const char* fmt, ...) {
 struct smaug_ctx {
   smaug_params p;
   Geo G;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // stream all work is queued on
+  hipStream_t own = nullptr;     // the context's own stream
   int nd = 0, np = 0, nb = 0, npr = 0, nvar = 0;
   long long N = 0;
   double* U[4] = {nullptr, nullptr, nullptr, nullptr};  // evolved-field buffers
@@ -72,6 +73,9 @@ struct smaug_ctx {
   double qt = 0;
   int it = 0;
   std::vector<double> h_pos, h_delx;
+  double* amap = nullptr;    // driver amplitude per (j,k) column
+  double* adj = nullptr;     // adjoint work: lam_s1..3, mu, T (np*N each), gcell (N)
+  double* aprim = nullptr;   // FP64 k_prim of an adjoint stage input
 };
 
 // ---- grid: positions extrapolated past the interior, delx (initgrid,
@@ -230,7 +234,10 @@ static void free_ctx(smaug_t* s) {
   hipFree(s->cmax);
   hipFree(s->dts);
   hipFree(s->flag);
-  if (s->stream) hipStreamDestroy(s->stream);
+  hipFree(s->amap);
+  hipFree(s->adj);
+  hipFree(s->aprim);
+  if (s->own) hipStreamDestroy(s->own);
   delete s;
 }
 
@@ -248,11 +255,12 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   s->nvar = 4 + 3 * s->nd;
   s->N = s->G.N;
   const size_t bytes = sizeof(double) * (size_t)s->N;
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  hipError_t e = hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete s;
     return fail(SMAUG_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
+  s->stream = s->own;
   const int nbuf = p->rkon == 1 ? 4 : 2;
   bool ok = true;
   for (int b = 0; b < nbuf; b++) ok &= hipMalloc(&s->U[b], bytes * s->np) == hipSuccess;
@@ -263,7 +271,7 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   ok &= hipMalloc(&s->delx, bytes * s->nd) == hipSuccess;
   ok &= hipMalloc(&s->rdelx, bytes * s->nd) == hipSuccess;
   ok &= hipMalloc(&s->pos, bytes * s->nd) == hipSuccess;
-  ok &= hipMalloc(&s->cmax, sizeof(unsigned long long) * 12) == hipSuccess;
+  ok &= hipMalloc(&s->cmax, sizeof(unsigned long long) * 16) == hipSuccess;
   ok &= hipMalloc(&s->dts, sizeof(double) * 2) == hipSuccess;
   ok &= hipMalloc(&s->flag, sizeof(int)) == hipSuccess;
   if (!ok) {
@@ -290,7 +298,7 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
     }
   }
   const double dts0[2] = {p->moddton == 1 ? 1.0e-8 : p->dt, 0.0};  // src/iosmaug.c:105,333-336
-  const unsigned long long z12[12] = {0};
+  const unsigned long long z12[16] = {0};
   const int z = 0;
   if (hipMemcpy(s->U[0], w0, bytes * s->np, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->bg, w0 + (size_t)s->np * s->N, bytes * s->nb, hipMemcpyHostToDevice) != hipSuccess ||
@@ -336,6 +344,7 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
   A.dts = s->dts;
   A.hfrac = hfrac;
   A.tdep = tdep;
+  A.amap = s->amap;
   return A;
 }
 
@@ -359,20 +368,33 @@ static double driver_tdep(const smaug_t* s) {
   return sin(2.0 * M_PI * s->qt / s->p.drv[1]);
 }
 
-static int step_once(smaug_t* s) {
+// One time step.  With a tape (device buffer of smaug_tape_doubles values)
+// the stage inputs are written into it (w^n, then the three intermediate
+// RK states, which the stages use in place of the context's buffers) and
+// the step's dt, qt and driver time factor follow them.
+static int step_once(smaug_t* s, double* tape = nullptr) {
   s->qt = s->qt + s->p.dt;  // p->qt += DT (src/iosmaug.c:636, quirk Q5)
   const double tdep = driver_tdep(s);
+  const long long F = (long long)s->np * s->N;
+  const size_t fbytes = sizeof(double) * (size_t)F;
   if (s->p.rkon == 0) {
     double* win = s->U[s->cur];
     double* wout = s->U[s->cur ^ 1];
+    if (tape) HIPCHK(hipMemcpyAsync(tape, win, fbytes, hipMemcpyDeviceToDevice, s->stream));
     run_stage(s, win, win, wout, 1.0, tdep, true, nullptr, nullptr, nullptr);
     s->cur ^= 1;
+    if (tape) launch_meta(s->stream, tape + F, s->dts, s->qt, tdep);
   } else {
-    double *w0 = s->U[0], *w1 = s->U[1], *w2 = s->U[2], *w3 = s->U[3];
+    double* w0 = s->U[0];
+    double* w1 = tape ? tape + F : s->U[1];
+    double* w2 = tape ? tape + 2 * F : s->U[2];
+    double* w3 = tape ? tape + 3 * F : s->U[3];
+    if (tape) HIPCHK(hipMemcpyAsync(tape, w0, fbytes, hipMemcpyDeviceToDevice, s->stream));
     run_stage(s, w0, w0, w1, 0.5, tdep, true, nullptr, nullptr, nullptr);
     run_stage(s, w1, w0, w2, 0.5, tdep, false, nullptr, nullptr, nullptr);
     run_stage(s, w2, w0, w3, 1.0, tdep, false, nullptr, nullptr, nullptr);
     run_stage(s, w3, w0, w0, 1.0 / 6.0, tdep, false, w1, w2, w3);
+    if (tape) launch_meta(s->stream, tape + 4 * F, s->dts, s->qt, tdep);
   }
   s->it = s->it + 1;
   HIPCHK(hipGetLastError());
@@ -496,6 +518,160 @@ extern "C" int smaug_get_clock(const smaug_t* cs, int* it, double* t, double* qt
 extern "C" int smaug_state_device_ptr(const smaug_t* s, void** dptr) {
   if (!s || !dptr) return fail(SMAUG_E_ARG, "null argument");
   *dptr = s->U[s->cur];
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_set_stream(smaug_t* s, void* stream) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->stream = stream ? (hipStream_t)stream : s->own;
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_set_clock(smaug_t* s, int it, double qt) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  s->it = it;
+  s->qt = qt;
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_save_pert(const smaug_t* cs, double* dst) {
+  smaug_t* s = const_cast<smaug_t*>(cs);
+  if (!s || !dst) return fail(SMAUG_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(dst, s->U[s->cur], sizeof(double) * (size_t)s->np * s->N, hipMemcpyDeviceToDevice,
+                        s->stream));
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_load_pert(smaug_t* s, const double* src) {
+  if (!s || !src) return fail(SMAUG_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(s->U[s->cur], src, sizeof(double) * (size_t)s->np * s->N, hipMemcpyDeviceToDevice,
+                        s->stream));
+  launch_fill(s->G, s->stream, s->U[s->cur]);
+  HIPCHK(hipGetLastError());
+  return SMAUG_OK;
+}
+
+// The differentiable path (driver amplitude map, taped steps, adjoint
+// steps) runs only when SMAUG_ADJOINT=1 is set in the environment: its
+// kernels are checked against the oracle in a host build but not yet on
+// MI355X, where its first run ended in a GPU fault whose cause is not
+// identified.
+static int adjoint_gate(const char* what) {
+  const char* e = getenv("SMAUG_ADJOINT");
+  if (e && !strcmp(e, "1")) return SMAUG_OK;
+  return fail(SMAUG_E_UNSUPPORTED, "%s is disabled unless SMAUG_ADJOINT=1 (not yet validated on MI355X)", what);
+}
+
+extern "C" int smaug_set_driver_map(smaug_t* s, const double* amap, int on_device) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  if (amap && adjoint_gate("smaug_set_driver_map")) return SMAUG_E_UNSUPPORTED;
+  const size_t bytes = sizeof(double) * (size_t)(s->N / s->G.n[0]);
+  if (!amap) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    hipFree(s->amap);
+    s->amap = nullptr;
+    return SMAUG_OK;
+  }
+  if (s->p.driver != SMAUG_DRIVER_VERTICAL)
+    return fail(SMAUG_E_ARG, "a driver amplitude map needs driver = SMAUG_DRIVER_VERTICAL");
+  if (!s->amap) HIPCHK(hipMalloc(&s->amap, bytes));
+  HIPCHK(hipMemcpyAsync(s->amap, amap, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        s->stream));
+  if (!on_device) HIPCHK(hipStreamSynchronize(s->stream));
+  return SMAUG_OK;
+}
+
+extern "C" long long smaug_tape_doubles(const smaug_t* s) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  return (long long)(s->p.rkon == 1 ? 4 : 1) * s->np * s->N + 8;
+}
+
+extern "C" int smaug_step_taped(smaug_t* s, double* tape) {
+  if (!s || !tape) return fail(SMAUG_E_ARG, "null argument");
+  if (adjoint_gate("smaug_step_taped")) return SMAUG_E_UNSUPPORTED;
+  return step_once(s, tape);
+}
+
+// Adjoint of one step (see include/smaug_hip.h).  Stage k of the forward
+// step maps its input s_{k-1} (s_0 = w^n) to s_k; the adjoint runs the
+// stages backwards, each as k_adj_mu -> k_prim(FP64) -> k_adj_rhs -> fold.
+extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* wnext, double* lam,
+                                  double* gmap) {
+  if (!s || !tape || !wnext || !lam) return fail(SMAUG_E_ARG, "null argument");
+  if (adjoint_gate("smaug_adjoint_step")) return SMAUG_E_UNSUPPORTED;
+  const long long N = s->N, F = (long long)s->np * N;
+  if (!s->adj) {
+    HIPCHK(hipMalloc(&s->adj, sizeof(double) * (size_t)(5 * F + N)));
+    HIPCHK(hipMalloc(&s->aprim, sizeof(double) * (size_t)(s->npr * N)));
+  }
+  double* L1 = s->adj;
+  double* L2 = L1 + F;
+  double* L3 = L2 + F;
+  double* MU = L3 + F;
+  double* T = MU + F;
+  double* gcell = T + F;
+  Geo Gd = s->G;
+  Gd.mixed = 0;  // the adjoint reads FP64 derived fields
+  hipStream_t st = s->stream;
+  unsigned long long* cmx = s->cmax + 12;
+  const int nst = s->p.rkon == 1 ? 4 : 1;
+  const double* meta = tape + nst * F;
+  auto stage = [&](const double* win, const double* lam_out, const double* outr, bool comb, double hfrac,
+                   double cb, int base_set, double* target, bool first) {
+    AdjMuArgs M;
+    memset(&M, 0, sizeof M);
+    M.lam_out = lam_out;
+    M.outr = outr;
+    if (comb) {
+      M.c0 = tape;
+      M.c1 = tape + F;
+      M.c2 = tape + 2 * F;
+      M.c3 = tape + 3 * F;
+      M.lam_r1 = L1;
+      M.lam_r2 = L2;
+      M.lam_r3 = L3;
+    }
+    M.bg = s->bg;
+    M.pos = s->pos;
+    M.amap = s->amap;
+    M.meta = meta;
+    M.hfrac = hfrac;
+    M.cb = cb;
+    M.base_set = base_set;
+    M.mu = MU;
+    M.lam_base = lam;
+    M.gcell = gcell;
+    M.gcell_set = first ? 1 : 0;
+    launch_adj_mu(s->G, st, M);
+    launch_zero3(st, cmx);
+    launch_prim(Gd, st, win, s->bg, s->aprim, nullptr, cmx, s->flag);
+    AdjArgs R;
+    R.win = win;
+    R.bg = s->bg;
+    R.prim = s->aprim;
+    R.delx = s->delx;
+    R.rdelx = s->rdelx;
+    R.dx1 = s->dx1;
+    R.rdx1 = s->rdx1;
+    R.mu = MU;
+    R.cmax_bits = cmx;
+    R.out = T;
+    launch_adj_rhs(s->G, st, R);
+    launch_adj_fold(s->G, st, T, target);
+  };
+  if (s->p.rkon == 0) {
+    stage(tape, lam, wnext, false, 1.0, 0.0, 1, lam, true);
+  } else {
+    const double *w0 = tape, *s1 = tape + F, *s2 = tape + 2 * F, *s3 = tape + 3 * F;
+    // final stage: out = w0 + h/6 acc(s3) + src, w^{n+1} = out + (s1 + 2 s2 + s3 - 4 w0)/3
+    stage(s3, lam, wnext, true, 1.0 / 6.0, -4.0 / 3.0, 1, L3, true);
+    stage(s2, L3, s3, false, 1.0, 0.0, 0, L2, false);
+    stage(s1, L2, s2, false, 0.5, 0.0, 0, L1, false);
+    stage(w0, L1, s1, false, 0.5, 0.0, 0, lam, false);
+  }
+  if (gmap) launch_colsum(s->G, st, gcell, gmap);
+  HIPCHK(hipGetLastError());
   return SMAUG_OK;
 }
 

@@ -1,0 +1,146 @@
+// smaug_ops.h -- device operators shared by the stage kernel (k_rhs) and
+// its adjoint (k_adj_rhs): grid index helpers, the hyperdiffusion halo
+// substitutions, the nu coefficient and the cell fluxes.
+#pragma once
+#include <math.h>
+
+#include "smaug_dev.h"
+
+namespace smaug {
+
+__device__ inline long long gix(const Geo& G, int a, int b, int c) {
+  return ((long long)c * G.n[1] + b) * G.n[0] + a;
+}
+
+template <int ND>
+__device__ inline bool in_grid(const Geo& G, const int* x) {
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if (x[d] < 0 || x[d] >= G.n[d]) return false;
+  return true;
+}
+
+template <int ND>
+__device__ inline bool interior_c(const Geo& G, const int* x) {
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if (x[d] < 2 || x[d] > G.n[d] - 3) return false;
+  return true;
+}
+
+// q at grid coordinates x, one step past the grid allowed in one direction:
+// coordinate -1 reads coordinate 5 (left-chain halo, quirk Q8), coordinate n
+// reads the right-chain stray halo (quirk Q7).
+template <int ND, typename SH>
+__device__ inline SH qval(const Geo& G, const SH* __restrict__ qa, const int* x) {
+  int out = -1, nout = 0;
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if (x[d] < 0 || x[d] >= G.n[d]) {
+      out = d;
+      nout++;
+    }
+  if (nout == 0) return qa[gix(G, x[0], x[1], ND == 3 ? x[2] : 0)];
+  if (nout > 1) return (SH)0;
+  int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
+  if (x[out] == -1) {
+    y[out] = 5;
+    return qa[gix(G, y[0], y[1], y[2])];
+  }
+  if (x[out] == G.n[out]) {
+    int pc[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) pc[q] = y[q] + 1;
+    pc[out] = G.n[out] + 1;
+    const long long h = halo_r_src(G, pc);
+    return h < 0 ? (SH)0 : qa[h];
+  }
+  return (SH)0;
+}
+
+// nu_{i+1/2}, nu_{i-1/2} from the 7 values q(i-3..i+3) along one direction:
+// third differences at 4 interfaces, first differences at 6, maxima over 3
+// and 5 interfaces, nu = dxk * max3 / max5 (src/hyperdifvisc1r.cu:426-708,
+// src/hyperdifvisc1l.cu:723-853); dxk = delx * cmax * chyp.
+__device__ inline void nu7(const double* Q, double dxk, double& nur, double& nul) {
+  double D3[5], D1[6];
+#pragma unroll
+  for (int a = 1; a <= 4; a++) D3[a] = fabs(3.0 * (Q[a + 1] - Q[a]) - (Q[a + 2] - Q[a - 1]));
+#pragma unroll
+  for (int a = 0; a <= 5; a++) D1[a] = fabs(Q[a + 1] - Q[a]);
+  // maxima over the right (i+1/2) and left (i-1/2) windows share their
+  // middle; fmax ignores NaN as the reference's "if (v > max)" does
+  const double t3 = fmax(D3[2], D3[3]);
+  const double t5 = fmax(fmax(D1[1], D1[2]), fmax(D1[3], D1[4]));
+  const double m3r = fmax(fmax(t3, D3[4]), 0.0), m3l = fmax(fmax(t3, D3[1]), 0.0);
+  const double m5r = fmax(fmax(t5, D1[5]), 0.0), m5l = fmax(fmax(t5, D1[0]), 0.0);
+  nur = m5r > 0 ? dxk * m3r / m5r : 0.0;
+  nul = m5l > 0 ? dxk * m3l / m5l : 0.0;
+}
+
+// nu from a window of q staged in LDS (centre index ci, stride s)
+template <typename SH>
+__device__ inline void nu_win(const SH* __restrict__ qb, int ci, int s, double dxk, double& nur, double& nul) {
+  double Q[7];
+#pragma unroll
+  for (int o = 0; o < 7; o++) Q[o] = qb[ci + (o - 3) * s];
+  nu7(Q, dxk, nur, nul);
+}
+
+// fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,
+// src/centdiff2.cu:158-356)
+template <int ND, typename SH>
+__device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
+                               int d, double* F) {
+  // v = m/rho_t and p_t are recomputed from the cell's state (4 fewer
+  // loads per flux cell than reading k_prim's copies); p_t follows
+  // computept3, src/dervfields.cu:107-153
+  const long long N = G.N;
+  double m[ND], b[ND], bb[ND];
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    m[q] = w[(long long)fMOM(q) * N + cc];
+    b[q] = w[(long long)fB(ND, q) * N + cc];
+    bb[q] = bg[(long long)bBB(q) * N + cc];
+  }
+  const double rho = w[cc], rhob = bg[(long long)bRHOB() * N + cc];
+  const double e = w[(long long)fE(ND) * N + cc], eb = bg[(long long)bEB() * N + cc];
+  const double rt = rho + rhob;
+  double v[ND];
+#pragma unroll
+  for (int q = 0; q < ND; q++) v[q] = m[q] / rt;
+  double mm = m[0] * m[0], b2 = b[0] * b[0], bbb = b[0] * bb[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++) {
+    mm = mm + m[q] * m[q];
+    b2 = b2 + b[q] * b[q];
+    bbb = bbb + b[q] * bb[q];
+  }
+  const double pt = (G.gamma - 1.0) * (e - 0.5 * mm / rt) - (G.gamma - 2.0) * (bbb + 0.5 * b2);
+  double vd = v[0], bd = b[0], bbd = bb[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++)
+    if (q == d) {
+      vd = v[q];
+      bd = b[q];
+      bbd = bb[q];
+    }
+  F[fRHO()] = vd * rho + rhob * vd;
+#pragma unroll
+  for (int f = 0; f < ND; f++) {
+    const double mag = -(b[f] * bbd + bb[f] * bd) - b[f] * bd;
+    F[fMOM(f)] = vd * m[f] + (f == d ? pt + mag : mag);
+  }
+  double bbv = bb[0] * v[0], bv = b[0] * v[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++) {
+    bbv = bbv + bb[q] * v[q];
+    bv = bv + b[q] * v[q];
+  }
+  F[fE(ND)] = vd * e + ((vd * pt - bd * bbv - bbd * bv) + vd * eb - bd * bv);
+#pragma unroll
+  for (int f = 0; f < ND; f++)
+    F[fB(ND, f)] = (f == d) ? 0.0 : vd * b[f] - (bd + bbd) * v[f] + bb[f] * vd;
+}
+
+}  // namespace smaug

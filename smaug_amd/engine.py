@@ -23,6 +23,8 @@ EXPORTS = [
     "smaug_set_state", "smaug_get_grid", "smaug_get_clock", "smaug_state_device_ptr",
     "smaug_sync", "smaug_destroy", "smaug_last_error", "smaug_version", "smaug_time_kernels",
     "smaug_ini_read", "smaug_ini_write", "smaug_out_write", "smaug_write_out",
+    "smaug_set_driver_map", "smaug_set_stream", "smaug_set_clock", "smaug_save_pert", "smaug_load_pert",
+    "smaug_tape_doubles", "smaug_step_taped", "smaug_adjoint_step",
 ]
 
 
@@ -59,9 +61,19 @@ def load() -> ctypes.CDLL:
                                   ctypes.c_double, ctypes.c_double, dp]
     L.smaug_out_write.argtypes = L.smaug_ini_write.argtypes
     L.smaug_write_out.argtypes = [vp, cp, cp]
+    L.smaug_set_driver_map.argtypes = [vp, vp, ctypes.c_int]
+    L.smaug_set_stream.argtypes = [vp, vp]
+    L.smaug_set_clock.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    L.smaug_save_pert.argtypes = [vp, vp]
+    L.smaug_load_pert.argtypes = [vp, vp]
+    L.smaug_tape_doubles.argtypes = [vp]
+    L.smaug_step_taped.argtypes = [vp, vp]
+    L.smaug_adjoint_step.argtypes = [vp, vp, vp, vp, vp]
     for name in EXPORTS:
-        getattr(L, name).restype = getattr(L, name).restype if name in (
-            "smaug_last_error", "smaug_version") else ctypes.c_int
+        getattr(L, name).restype = ctypes.c_int
+    L.smaug_last_error.restype = ctypes.c_char_p
+    L.smaug_version.restype = ctypes.c_char_p
+    L.smaug_tape_doubles.restype = ctypes.c_longlong
     _lib = L
     return L
 
@@ -138,6 +150,51 @@ class Engine:
         _chk(load().smaug_write_out(self._h, prefix.encode(), name.encode()), "smaug_write_out")
         return f"{prefix}_{self.clock()[0]}.out"
 
+    # ---- device-pointer entry points (buffers owned by the caller, e.g. torch)
+    @property
+    def npert(self) -> int:
+        return 2 + 2 * self.nd
+
+    def set_driver_map(self, amap=None, device_ptr: int = 0):
+        """Driver amplitude per (j, k) column: a host array, or a device
+        pointer (device_ptr) to n1*n2 doubles; None and 0 clear it."""
+        L = load()
+        if device_ptr:
+            _chk(L.smaug_set_driver_map(self._h, ctypes.c_void_p(device_ptr), 1), "smaug_set_driver_map")
+        elif amap is None:
+            _chk(L.smaug_set_driver_map(self._h, None, 0), "smaug_set_driver_map")
+        else:
+            a = np.ascontiguousarray(amap, dtype=np.float64).reshape(-1)
+            if a.size != self.N // self.rp.n[0]:
+                raise ValueError("driver map needs one value per (j, k) column")
+            _chk(L.smaug_set_driver_map(self._h, a.ctypes.data_as(ctypes.c_void_p), 0), "smaug_set_driver_map")
+
+    def set_stream(self, stream_ptr: int = 0):
+        _chk(load().smaug_set_stream(self._h, ctypes.c_void_p(stream_ptr or None)), "smaug_set_stream")
+
+    def set_clock(self, it: int, qt: float):
+        _chk(load().smaug_set_clock(self._h, int(it), float(qt)), "smaug_set_clock")
+
+    def save_pert(self, dst_ptr: int):
+        _chk(load().smaug_save_pert(self._h, ctypes.c_void_p(dst_ptr)), "smaug_save_pert")
+
+    def load_pert(self, src_ptr: int):
+        _chk(load().smaug_load_pert(self._h, ctypes.c_void_p(src_ptr)), "smaug_load_pert")
+
+    def tape_doubles(self) -> int:
+        n = int(load().smaug_tape_doubles(self._h))
+        if n < 0:
+            _chk(n, "smaug_tape_doubles")
+        return n
+
+    def step_taped(self, tape_ptr: int):
+        _chk(load().smaug_step_taped(self._h, ctypes.c_void_p(tape_ptr)), "smaug_step_taped")
+
+    def adjoint_step(self, tape_ptr: int, wnext_ptr: int, lam_ptr: int, gmap_ptr: int = 0):
+        _chk(load().smaug_adjoint_step(self._h, ctypes.c_void_p(tape_ptr), ctypes.c_void_p(wnext_ptr),
+                                       ctypes.c_void_p(lam_ptr), ctypes.c_void_p(gmap_ptr or None)),
+             "smaug_adjoint_step")
+
     def time_kernels(self, nsteps: int) -> Tuple[float, float, float]:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _chk(load().smaug_time_kernels(self._h, int(nsteps), ctypes.byref(a), ctypes.byref(b),
@@ -146,6 +203,8 @@ class Engine:
 
     def close(self):
         if getattr(self, "_h", None):
+            if getattr(self, "_torch_stream", None) is not None:
+                load().smaug_set_stream(self._h, None)  # back to the own stream first
             load().smaug_destroy(self._h)
             self._h = None
 

@@ -1,0 +1,88 @@
+"""Shared pieces of the adjoint tests: the line loss in numpy and central
+finite differences of the oracle with its hyperdiffusion coefficients
+replayed from an unperturbed run (the frozen-coefficient derivative the
+adjoint computes).  Directional derivatives are central differences at
+steps h and 2h (h relative to the direction's scale) combined by Richardson
+extrapolation, which removes the O(h^2) truncation error that the strongly
+nonlinear response leaves at h = 1e-6."""
+from __future__ import annotations
+
+import numpy as np
+
+from smaug_amd import params as P
+from smaug_amd.params import npert
+
+
+def line_loss_np(rp, w: np.ndarray, i_line: int) -> float:
+    """sum over interior (j, k) of (m_0 / (rho + rho_b))^2 at dim-0 index i_line."""
+    nd = rp.ndim
+    n = rp.n
+    shape = (w.shape[0],) + tuple(n[:nd][::-1])
+    v = w.reshape(shape)
+    inner = tuple(slice(2, n[d] - 2) for d in range(nd - 1, 0, -1))
+    irhob = 3 + 2 * nd
+    rho = v[0][inner + (i_line,)]
+    m0 = v[1][inner + (i_line,)]
+    rb = v[irhob][inner + (i_line,)]
+    vz = m0 / (rho + rb)
+    return float((vz * vz).sum())
+
+
+def interior_mask(rp) -> np.ndarray:
+    nd = rp.ndim
+    n = rp.n
+    m = np.zeros(tuple(n[:nd][::-1]), dtype=bool)
+    m[tuple(slice(2, n[d] - 2) for d in range(nd - 1, -1, -1))] = True
+    return m.ravel()
+
+
+class ReplayFD:
+    """L(w0, amap) = sum_{n=1..nsteps} line_loss(w_n) on the oracle, with the
+    nu coefficients recorded on the base run and replayed on every other."""
+
+    def __init__(self, oracle_mod, rp, w0, pos, amap, nsteps, i_line):
+        self.rp, self.nsteps, self.i_line = rp, nsteps, i_line
+        self.w0 = np.array(w0, dtype=np.float64).reshape(-1, rp.ncells)
+        self.amap = None if amap is None else np.array(amap, dtype=np.float64)
+        self.o = oracle_mod.Oracle(rp, self.w0, pos)
+        self.o.nu_mode(self.o.NU_RECORD)
+        self.base = self._run(self.w0, self.amap)
+        npairs = self.o.nu_mode(self.o.NU_REPLAY)
+        hyper_terms = rp.terms & (P.T_HRHO | P.T_HE | P.T_HMOM | P.T_HB)
+        assert npairs > 0 or not (rp.hyperdifmom and hyper_terms)
+
+    def _run(self, w0, amap) -> float:
+        o = self.o
+        o.restart(w0, self.rp.it0, self.rp.qt0)
+        if amap is not None:
+            o.set_driver_map(amap)
+        o.nu_mode(o.NU_REPLAY if hasattr(self, "base") else o.NU_RECORD)
+        total = 0.0
+        for _ in range(self.nsteps):
+            o.step(1)
+            total += line_loss_np(self.rp, o.state(), self.i_line)
+        return total
+
+    def _central(self, f, eps: float) -> float:
+        return (f(eps) - f(-eps)) / (2 * eps)
+
+    def _richardson(self, f, eps: float) -> float:
+        """Central differences at eps and 2 eps combined to cancel the
+        eps^2 term: (4 D(eps) - D(2 eps)) / 3, error O(eps^4)."""
+        return (4.0 * self._central(f, eps) - self._central(f, 2 * eps)) / 3.0
+
+    def dir_w0(self, delta: np.ndarray, eps: float) -> float:
+        k = npert(self.rp.ndim)
+
+        def f(e):
+            w = self.w0.copy()
+            w[:k] += e * delta
+            return self._run(w, self.amap)
+
+        return self._richardson(f, eps)
+
+    def dir_amap(self, delta: np.ndarray, eps: float) -> float:
+        return self._richardson(lambda e: self._run(self.w0, self.amap + e * delta), eps)
+
+    def close(self):
+        self.o.close()

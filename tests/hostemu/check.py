@@ -1,0 +1,188 @@
+"""TEST INFRASTRUCTURE: runs one check of the engine compiled for the host
+(tests/hostemu/build.py) and prints its result as one JSON line.  Started
+as a subprocess by tests/test_hostemu.py with SMAUG_HIP_LIB pointing at the
+host build, so the product binding (smaug_amd.engine) drives the same C ABI
+and kernels the GPU build compiles.
+
+  python -m tests.hostemu.check fwd  <case>            forward vs oracle
+  python -m tests.hostemu.check adj  <case> <terms>    adjoint vs oracle FD
+  python -m tests.hostemu.check autograd <rkon>        smaug_amd.adjoint path
+
+case: s2r (2D stratified RK4), s2e (2D Euler), s3r (3D RK4), ot (2D periodic).
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from smaug_amd import params as P  # noqa: E402
+from smaug_amd.engine import Engine  # noqa: E402
+from tests.adjoint_cases import ReplayFD, interior_mask  # noqa: E402
+from tests.cases import compare, ot_case, strat_case  # noqa: E402
+import oracle.pyoracle as oracle  # noqa: E402
+
+
+def make_case(name: str, terms: int = P.TERMS_ALL):
+    if name == "ot":
+        rp, w, pos = ot_case(ni=16, rkon=1)
+        rp.moddton = 0
+        rp.dt = 2e-3
+        amap, il, nsteps = None, 9, 3
+    else:
+        nd = 3 if name.startswith("s3") else 2
+        rkon = 0 if name.endswith("e") else 1
+        rp, w, pos = strat_case(ni=(16, 12) if nd == 2 else (12, 8, 8), ndim=nd, rkon=rkon)
+        amap = np.random.default_rng(3).uniform(150.0, 250.0, rp.ncells // rp.n[0])
+        il, nsteps = (7, 3) if nd == 2 else (6, 2)
+    rp.terms = terms
+    return rp, w, pos, amap, il, nsteps
+
+
+def engine_adjoint(rp, w0, pos, amap, nsteps, i_line):
+    """Taped steps and adjoint steps through the C ABI, host buffers as the
+    "device" pointers (the host build's memory is host memory)."""
+    nd = rp.ndim
+    k = P.npert(nd)
+    e = Engine(rp, w0, pos)
+    if amap is not None:
+        e.set_driver_map(amap)
+    F = k * rp.ncells
+    T = e.tape_doubles()
+    tapes = np.full((nsteps, T), np.nan)
+    for n in range(nsteps):
+        e.step_taped(tapes[n].ctypes.data)
+    wend = np.zeros(F)
+    e.save_pert(wend.ctypes.data)
+    lam = np.zeros(F)
+    gmap = np.zeros(rp.ncells // rp.n[0])
+    shape = tuple(rp.n[:nd][::-1])
+    rhob = w0[3 + 2 * nd].reshape(shape)
+    inner = tuple(slice(2, rp.n[d] - 2) for d in range(nd - 1, 0, -1)) + (i_line,)
+    for n in reversed(range(nsteps)):
+        wn = wend if n == nsteps - 1 else tapes[n + 1, :F].copy()
+        v = wn.reshape((k,) + shape)
+        rt = v[0][inner] + rhob[inner]
+        vz = v[1][inner] / rt
+        L = lam.reshape(v.shape)
+        L[1][inner] += 2 * vz / rt
+        L[0][inner] += -2 * vz * vz / rt
+        e.adjoint_step(tapes[n].ctypes.data, wn.ctypes.data, lam.ctypes.data, gmap.ctypes.data)
+    e.close()
+    return lam.reshape(k, -1), gmap
+
+
+def check_fwd(case: str):
+    rp, w, pos, amap, _, nsteps = make_case(case)
+    e = Engine(rp, w, pos)
+    if amap is not None:
+        e.set_driver_map(amap)
+    e.step(nsteps)
+    got = e.state()
+    e.close()
+    o = oracle.Oracle(rp, w, pos)
+    if amap is not None:
+        o.set_driver_map(amap)
+    o.step(nsteps)
+    ref = o.state()
+    o.close()
+    return {"rel_l2": compare(rp, got, ref).tolist()}
+
+
+def check_adj(case: str, terms: int):
+    rp, w, pos, amap, il, nsteps = make_case(case, terms)
+    gw, ga = engine_adjoint(rp, w, pos, amap, nsteps, il)
+    fd = ReplayFD(oracle, rp, w, pos, amap, nsteps, il)
+    rng = np.random.default_rng(7)
+    k = P.npert(rp.ndim)
+    inner = interior_mask(rp)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1)) + 1e-30
+    out = {"fields": []}
+    for f in range(k):  # one direction per evolved field
+        delta = np.zeros((k, rp.ncells))
+        delta[f] = rng.standard_normal(rp.ncells) * scale[f] * inner
+        got, want = float((gw * delta).sum()), fd.dir_w0(delta, 1e-6)
+        out["fields"].append([got, want])
+    if amap is not None:
+        da = rng.standard_normal(amap.size) * np.abs(amap).mean()
+        out["amap"] = [float((ga * da).sum()), fd.dir_amap(da, 1e-6)]
+    fd.close()
+    return out
+
+
+def check_autograd(rkon: int):
+    """smaug_amd.adjoint.driver_loss on CPU tensors (the host build's device
+    memory is host memory); torch's stream calls are replaced by no-ops."""
+    import contextlib
+
+    import torch
+
+    class _Stream:
+        cuda_stream = 0x1
+
+        def wait_stream(self, other):
+            pass
+
+        def synchronize(self):
+            pass
+
+    st = _Stream()
+    torch.cuda.current_stream = lambda *a, **k: st
+    torch.cuda.Stream = lambda *a, **k: st
+    torch.cuda.stream = lambda s: contextlib.nullcontext()
+    torch.cuda.synchronize = lambda *a, **k: None
+    torch.Tensor.record_stream = lambda self, s: None
+    torch.Tensor.is_cuda = property(lambda self: True)
+    from smaug_amd.adjoint import driver_loss, line_vz2_loss
+
+    rp, w0, pos = strat_case(ni=(16, 12), ndim=2, rkon=rkon)
+    amap = np.random.default_rng(3).uniform(150.0, 250.0, rp.ncells // rp.n[0])
+    k = P.npert(2)
+    nsteps, il = 5, 7
+    res = {}
+    for every in (5, 2, 1):
+        eng = Engine(rp, w0, pos)
+        w0t = torch.tensor(np.ascontiguousarray(w0[:k]).ravel(), dtype=torch.float64, requires_grad=True)
+        amt = torch.tensor(amap, dtype=torch.float64, requires_grad=True)
+        rhob = torch.tensor(w0[3 + 4], dtype=torch.float64)
+        loss = driver_loss(eng, amt, w0t, nsteps, line_vz2_loss(eng, rhob, il), every=every,
+                           clock0=(rp.it0, rp.qt0))
+        loss.backward()
+        res[every] = (float(loss.item()), w0t.grad.numpy().copy(), amt.grad.numpy().copy())
+        eng.close()
+    fd = ReplayFD(oracle, rp, w0, pos, amap, nsteps, il)
+    rng = np.random.default_rng(7)
+    inner = interior_mask(rp)
+    scale = np.sqrt((w0[:k, inner] ** 2).mean(axis=1)) + 1e-30
+    delta = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    da = rng.standard_normal(amap.size) * amap.mean()
+    gw, ga = res[2][1].reshape(k, -1), res[2][2]
+    out = {
+        "loss": res[2][0], "oracle_loss": fd.base,
+        "w0": [float((gw * delta).sum()), fd.dir_w0(delta, 1e-6)],
+        "amap": [float((ga * da).sum()), fd.dir_amap(da, 1e-6)],
+        "every_identical": all(np.array_equal(res[e][1], res[5][1]) and np.array_equal(res[e][2], res[5][2])
+                               for e in (2, 1)),
+    }
+    fd.close()
+    return out
+
+
+def main(argv):
+    what = argv[0]
+    if what == "fwd":
+        r = check_fwd(argv[1])
+    elif what == "adj":
+        r = check_adj(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
+    else:
+        r = check_autograd(int(argv[1]))
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
