@@ -134,6 +134,27 @@ class _DriverLoss(torch.autograd.Function):
         return gmap, lam, None, None, None, None, None, None
 
 
+def _hip_runtimes() -> set:
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as fh:
+            return {ln.split()[-1] for ln in fh if "libamdhip64" in ln}
+    except OSError:
+        return set()
+
+
+def _check_shared_runtime():
+    """The engine and torch must use one HIP runtime for a torch stream
+    handle to mean anything to the engine.  That holds when torch is
+    imported before the engine library is first loaded (the library then
+    binds to torch's libamdhip64 by soname); loaded the other way round the
+    process holds two runtimes."""
+    libs = _hip_runtimes()
+    if len(libs) > 1:
+        raise RuntimeError("two HIP runtimes are loaded (%s): import torch before creating the first "
+                           "smaug_amd Engine so both use one runtime" % ", ".join(sorted(libs)))
+
+
 def _engine_stream(eng: Engine) -> "torch.cuda.Stream":
     """A torch stream (default priority) shared by the engine and the torch
     work of the loss; kept on the engine so it outlives every queued call."""
@@ -167,6 +188,7 @@ def driver_loss(eng: Engine, amap: torch.Tensor, w0: torch.Tensor, nsteps: int, 
         it, _, qt, _ = eng.clock()
         clock0 = (it, qt)
     every = int(every or checkpoint_every(nsteps))
+    _check_shared_runtime()
     cur = torch.cuda.current_stream()
     side = _engine_stream(eng)
     side.wait_stream(cur)
