@@ -60,6 +60,7 @@ struct orc {
   double cmax_dim[3];
   int cur;        /* slot holding the current state           */
   double* amap;   /* driver amplitude per column (n1*n2), or NULL */
+  double* fixed;  /* evolved fields of the initial state: fixed-boundary values */
   int nu_mode;    /* 0 compute nu, 1 compute and record, 2 replay */
   double* nu_tape;
   long nu_len, nu_cap, nu_pos;
@@ -994,24 +995,41 @@ static void k_source(orc_t* o, int sn) {
 
 /* ---------------- boundaries ------------------------------------------ */
 
-/* periodic (src/boundary.cu:72-225, 2D) and contcd4
- * (models/boundary.cu.fluxtube1:16-85,168-238,323-381) per field and dir */
+/* One boundary pass for field f along dir, in place, as the reference's
+ * per-direction helpers do it: periodic (src/boundary.cu:72-225, 2D),
+ * contcd4 (models/boundary.cu.fluxtube1:16-85,168-238,323-381), cont
+ * (bc3_cont_dir, src/gradops.cu:947-1060), fixed (bc3_fixed_dir,
+ * src/gradops.cu:1234-1268: the values the initialisation pass of
+ * src/iosmaug.c:416-425 means to capture, i.e. the initial state's),
+ * symm/asymm (bc3_symm_dir/bc3_asymm_dir, src/gradops.cu:1325-1466, whose
+ * 2D and 3D maps differ). */
 static void k_bound(orc_t* o, int s, int f, int dir) {
   double* a = W(o, s, f);
   const int bt = o->p.boundtype[f][dir][0];
   const int nn = nof(o, dir);
   const long st = stride(o, dir);
-  if (bt != 0 && bt != 4) return;
   FOR_ALL(o) {
     const int x = coord(i, j, k, dir);
     const long c = enc(o, i, j, k);
+    if (x > 1 && x < nn - 2) continue;
     if (bt == 0) {
       if (x == 0 || x == 1) a[c] = a[c + (nn - 4) * st];
-      else if (x == nn - 1 || x == nn - 2) a[c] = a[c - (nn - 4) * st];
-    } else {
+      else a[c] = a[c - (nn - 4) * st];
+    } else if (bt == 4) {
       if (x == 0 || x == 1) a[c] = a[c + (4 - 2 * x) * st];
       else if (x == nn - 1) a[c] = a[c - 4 * st];
-      else if (x == nn - 2) a[c] = a[c - 2 * st];
+      else a[c] = a[c - 2 * st];
+    } else if (bt == 3) {
+      if (x == 0 || x == 1) a[c] = a[c + (2 - x) * st];
+      else a[c] = a[c - (x - (nn - 3)) * st];
+    } else if (bt == 5) {
+      a[c] = o->fixed[(long)f * o->N + c];
+    } else if (bt == 6 || bt == 7) {
+      const double sg = bt == 7 ? -1.0 : 1.0;
+      long src;
+      if (o->nd == 3) src = (x == 0) ? 4 : (x == 1) ? 3 : (x == nn - 1) ? nn - 5 : nn - 4;
+      else src = (x == 0) ? 3 : (x == 1) ? 2 : (x == nn - 1) ? nn - 4 : nn - 3;
+      a[c] = sg * a[c + (src - x) * st];
     }
   }
 }
@@ -1255,7 +1273,7 @@ int orc_create(const smaug_params* p, const double* w0, const double* pos, orc_t
   for (int f = 0; f < SMAUG_MAXVAR; f++)
     for (int d = 0; d < o->nd; d++) {
       int bt = p->boundtype[f][d][0];
-      if (bt != 4 && !(bt == 0 && o->nd == 2)) {
+      if (bt == 1 || bt == 2 || bt < 0 || bt > 7 || (bt == 0 && o->nd == 3)) {
         if (f < 2 + 2 * o->nd) {
           free(o);
           return -5;
@@ -1279,6 +1297,8 @@ int orc_create(const smaug_params* p, const double* w0, const double* pos, orc_t
     return -3;
   }
   for (int s = 0; s < o->nslot; s++) memcpy(W(o, s, 0), w0, sizeof(double) * o->nvar * o->N);
+  o->fixed = (double*)malloc(sizeof(double) * (2 + 2 * o->nd) * o->N);
+  memcpy(o->fixed, w0, sizeof(double) * (2 + 2 * o->nd) * o->N);
   initgrid(o, pos);
   o->dt = p->moddton == 1 ? 1.0e-8 : p->dt; /* src/iosmaug.c:105,333-336 */
   o->qt = p->qt0;
@@ -1302,6 +1322,7 @@ int orc_get_state(orc_t* o, double* w) {
 
 int orc_set_state(orc_t* o, const double* w) {
   memcpy(W(o, o->cur, 0), w, sizeof(double) * o->nvar * o->N);
+  memcpy(o->fixed, w, sizeof(double) * (2 + 2 * o->nd) * o->N);
   return 0;
 }
 
@@ -1361,6 +1382,7 @@ int orc_probe_visc(orc_t* o, int field, int dim, double* nur, double* nul, doubl
 /* restart from state w (nvar*N) at step it and clock qt, as orc_create */
 int orc_restart(orc_t* o, const double* w, int it, double qt) {
   for (int s = 0; s < o->nslot; s++) memcpy(W(o, s, 0), w, sizeof(double) * o->nvar * o->N);
+  memcpy(o->fixed, w, sizeof(double) * (2 + 2 * o->nd) * o->N);
   o->dt = o->p.moddton == 1 ? 1.0e-8 : o->p.dt;
   o->qt = qt;
   o->it = it;
@@ -1409,6 +1431,7 @@ void orc_destroy(orc_t* o) {
   free(o->wt2);
   free(o->dwn1);
   free(o->amap);
+  free(o->fixed);
   free(o->nu_tape);
   free(o);
 }

@@ -574,8 +574,9 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// k_adj_fold: lam(p) += T(p) + T(ghost images of p), interior p, per field
-// with that field's boundary map (the transpose of the ghost fill).
+// k_adj_fold: lam(p) += T(p) + sign * T(ghost images of p), interior p, per
+// field with that field's boundary map (the transpose of the ghost fill; a
+// fixed ghost is a constant and passes nothing on).
 template <int ND>
 __global__ __launch_bounds__(256) void k_adj_fold(Geo G, const double* __restrict__ Tm, double* __restrict__ lam) {
   const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -591,11 +592,12 @@ __global__ __launch_bounds__(256) void k_adj_fold(Geo G, const double* __restric
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       int im[4];
-      const int m = bc_images(G.bct[f][d], x[d], G.n[d], im);
+      const int m = bc_images(G.bct[f][d], x[d], G.n[d], ND, im);
+      const double sg = bc_sign(G.bct[f][d]);
       for (int q = 0; q < m; q++) {
         int y[3] = {x[0], x[1], x[2]};
         y[d] = im[q];
-        s += Tf[gix(G, y[0], y[1], y[2])];
+        s += sg * Tf[gix(G, y[0], y[1], y[2])];
       }
     }
     lam[(long long)f * G.N + c] += s;

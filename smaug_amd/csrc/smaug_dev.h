@@ -28,6 +28,7 @@ struct Geo {
   double chyp[kMaxPert]; // hyperdiffusion coefficient per evolved field
   unsigned zm[kMaxVar];  // bit d: boundtype[idx][d][0] != 0 (gradient zeroing)
   int bct[kMaxPert][3];  // boundary type per evolved field and dir
+  int has_fixed;         // some evolved field has a fixed (5) boundary
   int bc_uniform;        // 1 when every evolved field has the same types
   unsigned terms;        // SMAUG_T_* mask
   int hyper;             // PHYPERDIFMOM
@@ -58,37 +59,48 @@ __host__ __device__ inline int zVEL(int d) { return d; }     // DEV vel1+d
 constexpr int zRHO = 0, zTMP1 = 0, zTMP3 = 2, zTMP4 = 3, zTMP5 = 4, zTMP7 = 6, zTMP8 = 7;
 
 // ---------------------------------------------------------------------------
-// boundary index maps (periodic: src/boundary.cu:132-136,209-213;
-// contcd4: models/boundary.cu.fluxtube1:59-64)
-// For ghost coordinate x returns the interior coordinate it copies.
-__host__ __device__ inline int bc_src(int type, int x, int n) {
-  if (type == 0) {  // periodic
-    if (x < 2) return x + n - 4;
-    if (x >= n - 2) return x - (n - 4);
-    return x;
+// boundary index maps, per dim, for ghost width 2:
+//   0 periodic  g <- g + (n-4) low, g - (n-4) high      (src/boundary.cu:132-136,209-213)
+//   3 cont      0,1 <- 2;  n-2,n-1 <- n-3               (src/gradops.cu:947-1060)
+//   4 contcd4   0 <- 4, 1 <- 3, n-1 <- n-5, n-2 <- n-4  (models/boundary.cu.fluxtube1:59-64)
+//   5 fixed     the ghost keeps the value the initial state gives it
+//               (src/gradops.cu:1201-1268, src/iosmaug.c:416-425)
+//   6 symm      mirror: 3D as contcd4; 2D 0 <- 3, 1 <- 2, n-1 <- n-4, n-2 <- n-3
+//   7 asymm     the symm map with the sign flipped    (src/gradops.cu:1325-1466)
+constexpr int BC_PERIODIC = 0, BC_CONT = 3, BC_CONTCD4 = 4, BC_FIXED = 5, BC_SYMM = 6, BC_ASYMM = 7;
+
+__host__ __device__ inline bool bc_is_ghost(int x, int n) { return x < 2 || x > n - 3; }
+__host__ __device__ inline double bc_sign(int type) { return type == BC_ASYMM ? -1.0 : 1.0; }
+
+// interior coordinate the ghost coordinate x copies (x itself when x is
+// not a ghost, -1 for a fixed ghost)
+__host__ __device__ inline int bc_src(int type, int x, int n, int nd) {
+  if (!bc_is_ghost(x, n)) return x;
+  switch (type) {
+    case BC_PERIODIC:
+      return x < 2 ? x + n - 4 : x - (n - 4);
+    case BC_CONT:
+      return x < 2 ? 2 : n - 3;
+    case BC_FIXED:
+      return -1;
+    case BC_SYMM:
+    case BC_ASYMM:
+      if (nd == 2) return x < 2 ? 3 - x : 2 * n - 5 - x;  // 0<-3 1<-2 n-2<-n-3 n-1<-n-4
+      return x < 2 ? 4 - x : 2 * n - 6 - x;                // 0<-4 1<-3 n-2<-n-4 n-1<-n-5
+    default:  // contcd4
+      return x < 2 ? 4 - x : 2 * n - 6 - x;
   }
-  // contcd4: 0<-4, 1<-3, n-1<-n-5, n-2<-n-4
-  if (x < 2) return 4 - x;
-  if (x == n - 1) return n - 5;
-  if (x == n - 2) return n - 4;
-  return x;
 }
 
-// Ghost coordinates whose value is copied from interior coordinate y
-// (at most two per dim); returns how many were written to out[].
-__host__ __device__ inline int bc_images(int type, int y, int n, int* out) {
+// ghost coordinates whose value is copied from interior coordinate y
+// (at most four per dim); returns how many were written to out[]
+__host__ __device__ inline int bc_images(int type, int y, int n, int nd, int* out) {
   int m = 0;
-  if (type == 0) {
-    if (y == n - 4) out[m++] = 0;
-    if (y == n - 3) out[m++] = 1;
-    if (y == 2) out[m++] = n - 2;
-    if (y == 3) out[m++] = n - 1;
-  } else {
-    if (y == 4) out[m++] = 0;
-    if (y == 3) out[m++] = 1;
-    if (y == n - 5) out[m++] = n - 1;
-    if (y == n - 4) out[m++] = n - 2;
-  }
+  if (type == BC_FIXED) return 0;
+  const int g[4] = {0, 1, n - 2, n - 1};
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (bc_src(type, g[q], n, nd) == y) out[m++] = g[q];
   return m;
 }
 

@@ -160,8 +160,9 @@ static int check_params(const smaug_params* p) {
       const int bt = p->boundtype[f][d][0];
       if (bt == 0 && p->ndim == 3)
         return fail(SMAUG_E_UNSUPPORTED, "periodic boundaries are supported in 2D only");
-      if (bt != 0 && bt != 4)
-        return fail(SMAUG_E_UNSUPPORTED, "boundary type %d not supported (0 and 4 are)", bt);
+      if (bt == 1 || bt == 2)
+        return fail(SMAUG_E_UNSUPPORTED, "MPI boundary types (1, 2) need a multi-domain run");
+      if (bt < 0 || bt > 7) return fail(SMAUG_E_ARG, "unknown boundary type %d", bt);
     }
   if (p->moddton == 1 && !(p->courant > 0)) return fail(SMAUG_E_ARG, "courant must be > 0");
   if (!(p->dt > 0)) return fail(SMAUG_E_ARG, "dt must be > 0");
@@ -193,6 +194,7 @@ static void make_geo(const smaug_params* p, Geo& G) {
     for (int d = 0; d < 3; d++) {
       G.bct[f][d] = d < p->ndim ? p->boundtype[f][d][0] : 4;
       if (G.bct[f][d] != G.bct[0][d]) G.bc_uniform = 0;
+      if (d < p->ndim && G.bct[f][d] == BC_FIXED) G.has_fixed = 1;
     }
   G.terms = p->terms;
   G.hyper = p->hyperdifmom;
@@ -315,6 +317,13 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   s->it = p->it0;
   s->cur = 0;
   launch_fill(s->G, s->stream, s->U[0]);  // src/iosmaug.c:480-488
+  // every buffer a stage writes starts as the initial state, so ghost cells
+  // no stage writes (fixed boundaries) hold their initial values in each
+  for (int b = 1; b < nbuf; b++)
+    if (hipMemcpyAsync(s->U[b], s->U[0], bytes * s->np, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+      free_ctx(s);
+      return fail(SMAUG_E_HIP, "initial copy of the stage buffers failed");
+    }
   e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
     free_ctx(s);
@@ -390,6 +399,9 @@ static int step_once(smaug_t* s, double* tape = nullptr) {
     double* w2 = tape ? tape + 2 * F : s->U[2];
     double* w3 = tape ? tape + 3 * F : s->U[3];
     if (tape) HIPCHK(hipMemcpyAsync(tape, w0, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    if (tape && s->G.has_fixed)  // fixed ghosts of the tape's stage buffers
+      for (int k = 1; k <= 3; k++)
+        HIPCHK(hipMemcpyAsync(tape + k * F, w0, fbytes, hipMemcpyDeviceToDevice, s->stream));
     run_stage(s, w0, w0, w1, 0.5, tdep, true, nullptr, nullptr, nullptr);
     run_stage(s, w1, w0, w2, 0.5, tdep, false, nullptr, nullptr, nullptr);
     run_stage(s, w2, w0, w3, 1.0, tdep, false, nullptr, nullptr, nullptr);
@@ -471,11 +483,25 @@ extern "C" int smaug_get_state(const smaug_t* cs, double* w) {
   return check_flag(s);
 }
 
+// fixed boundaries: a new current state brings new ghost values, which every
+// stage buffer must hold (no stage writes them)
+static int copy_fixed_ghosts(smaug_t* s) {
+  if (!s->G.has_fixed) return SMAUG_OK;
+  const int nbuf = s->p.rkon == 1 ? 4 : 2;
+  for (int b = 0; b < nbuf; b++)
+    if (b != s->cur)
+      HIPCHK(hipMemcpyAsync(s->U[b], s->U[s->cur], sizeof(double) * (size_t)s->N * s->np, hipMemcpyDeviceToDevice,
+                            s->stream));
+  return SMAUG_OK;
+}
+
 extern "C" int smaug_set_state(smaug_t* s, const double* w) {
   if (!s || !w) return fail(SMAUG_E_ARG, "null argument");
   const size_t bytes = sizeof(double) * (size_t)s->N;
   HIPCHK(hipMemcpyAsync(s->U[s->cur], w, bytes * s->np, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(s->bg, w + (size_t)s->np * s->N, bytes * s->nb, hipMemcpyHostToDevice, s->stream));
+  int rc = copy_fixed_ghosts(s);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   return SMAUG_OK;
 }
@@ -549,7 +575,7 @@ extern "C" int smaug_load_pert(smaug_t* s, const double* src) {
                         s->stream));
   launch_fill(s->G, s->stream, s->U[s->cur]);
   HIPCHK(hipGetLastError());
-  return SMAUG_OK;
+  return copy_fixed_ghosts(s);
 }
 
 // The differentiable path (driver amplitude map, taped steps, adjoint

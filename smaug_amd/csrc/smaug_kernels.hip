@@ -146,8 +146,12 @@ __global__ void k_dt(const unsigned long long* cmax_bits, int nd, double dx0, do
 }
 
 // Ghost fill of the evolved fields (cuboundary for every field and dir,
-// src/iosmaug.c:830-833): after the reference's dir-by-dir passes a ghost
-// cell holds the interior cell reached by mapping each coordinate.
+// src/iosmaug.c:830-833).  The reference applies the directions in order
+// 0, 1, 2, each pass overwriting the ghost layers of its direction, so a
+// ghost cell ends with the value of the last direction in which it is a
+// ghost: a fixed ghost keeps its own value, an image-type ghost takes the
+// cell its map names as it stood before that pass (the remaining
+// directions), with the sign of asymm maps.
 template <int ND>
 __global__ __launch_bounds__(256) void k_fill(Geo G, double* __restrict__ w) {
   const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -157,13 +161,20 @@ __global__ __launch_bounds__(256) void k_fill(Geo G, double* __restrict__ w) {
   x[1] = (int)((c / G.n[0]) % G.n[1]);
   x[2] = ND == 3 ? (int)(c / ((long long)G.n[0] * G.n[1])) : 0;
   bool ghost = false;
-  for (int d = 0; d < ND; d++) ghost |= (x[d] < 2 || x[d] > G.n[d] - 3);
+  for (int d = 0; d < ND; d++) ghost |= bc_is_ghost(x[d], G.n[d]);
   if (!ghost) return;
   for (int f = 0; f < G.np; f++) {
-    int y[3] = {0, 0, 0};
-    for (int d = 0; d < ND; d++) y[d] = bc_src(G.bct[f][d], x[d], G.n[d]);
+    int y[3] = {x[0], x[1], x[2]};
+    double sg = 1.0;
+    for (int d = ND - 1; d >= 0; d--) {
+      if (!bc_is_ghost(y[d], G.n[d])) continue;
+      const int t = G.bct[f][d];
+      if (t == BC_FIXED) break;  // the value the cell holds
+      y[d] = bc_src(t, y[d], G.n[d], ND);
+      sg *= bc_sign(t);
+    }
     const long long src = ((long long)y[2] * G.n[1] + y[1]) * G.n[0] + y[0];
-    w[(long long)f * G.N + c] = w[(long long)f * G.N + src];
+    if (src != c) w[(long long)f * G.N + c] = sg * w[(long long)f * G.N + src];
   }
 }
 

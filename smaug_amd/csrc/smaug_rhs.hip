@@ -497,23 +497,30 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
     }
   }
   double* __restrict__ wo = A.wout;
+  // ghost images: every combination of this cell's images across the
+  // directions (a fixed direction has none), with the asymm signs
 #pragma unroll
   for (int f = 0; f < NP; f++) {
     if (f > 0 && G.bc_uniform) break;
-    int li[3][3], ln[3] = {1, 1, 1};
+    int li[3][5], ln[3] = {1, 1, 1};
+    double sgn[3] = {1.0, 1.0, 1.0};
 #pragma unroll
     for (int d = 0; d < 3; d++) li[d][0] = x[d];
 #pragma unroll
-    for (int d = 0; d < ND; d++) ln[d] = 1 + bc_images(G.bct[f][d], x[d], G.n[d], &li[d][1]);
+    for (int d = 0; d < ND; d++) {
+      ln[d] = 1 + bc_images(G.bct[f][d], x[d], G.n[d], ND, &li[d][1]);
+      sgn[d] = bc_sign(G.bct[f][d]);
+    }
     for (int a = 0; a < ln[0]; a++)
       for (int b2 = 0; b2 < ln[1]; b2++)
         for (int c2 = 0; c2 < ln[2]; c2++) {
           const long long cc = gix(G, li[0][a], li[1][b2], li[2][c2]);
+          const double sg = (a ? sgn[0] : 1.0) * (b2 ? sgn[1] : 1.0) * (c2 ? sgn[2] : 1.0);
           if (G.bc_uniform) {
 #pragma unroll
-            for (int q = 0; q < NP; q++) wo[(long long)q * N + cc] = out[q];
+            for (int q = 0; q < NP; q++) wo[(long long)q * N + cc] = sg * out[q];
           } else {
-            wo[(long long)f * N + cc] = out[f];
+            wo[(long long)f * N + cc] = sg * out[f];
           }
         }
   }

@@ -105,6 +105,9 @@ class RunParams:
     chyp: Dict[str, float] = field(default_factory=dict)
     bcu: List[int] = field(default_factory=lambda: [0, 0, 0])   # BCU0..2
     bcl: List[int] = field(default_factory=lambda: [0, 0, 0])   # BCL0..2
+    # per-field boundary types per dim, overriding bcu/bcl for the named
+    # fields (the reference's boundtype[field][dir] table, include/iotypes.h:144)
+    bc_field: Dict[str, List[int]] = field(default_factory=dict)
     driver: int = DRIVER_NONE
     drv: List[float] = field(default_factory=lambda: [0.0] * 8)
     terms: int = TERMS_ALL
@@ -147,10 +150,12 @@ class RunParams:
         for d in range(self.ndim):
             p.xmin[d], p.xmax[d] = self.xmin[d], self.xmax[d]
             p.dx[d] = self.dx[d]
+        all_names = field_names(self.ndim)
         for f in range(13):
+            over = self.bc_field.get(all_names[f]) if f < len(all_names) else None
             for d in range(3):
-                p.boundtype[f][d][0] = self.bcu[d] if d < self.ndim else 0
-                p.boundtype[f][d][1] = self.bcl[d] if d < self.ndim else 0
+                p.boundtype[f][d][0] = (over[d] if over else self.bcu[d]) if d < self.ndim else 0
+                p.boundtype[f][d][1] = (over[d] if over else self.bcl[d]) if d < self.ndim else 0
         p.driver = self.driver
         for i in range(8):
             p.drv[i] = self.drv[i] if i < len(self.drv) else 0.0

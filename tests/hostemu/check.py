@@ -29,6 +29,15 @@ import oracle.pyoracle as oracle  # noqa: E402
 
 
 def make_case(name: str, terms: int = P.TERMS_ALL):
+    if name.startswith("bc:"):  # bc:<tests.test_boundaries case>:<rkon>
+        from tests.test_boundaries import CASES, bc_case
+
+        _, key, rkon = name.split(":")
+        nd, types, ft = CASES[key]
+        rp, w, pos = bc_case(nd, int(rkon), types, ft)
+        amap = np.random.default_rng(3).uniform(150.0, 250.0, rp.ncells // rp.n[0])
+        rp.terms = terms
+        return rp, w, pos, amap, (7 if nd == 2 else 6), (3 if nd == 2 else 2)
     if name == "ot":
         rp, w, pos = ot_case(ni=16, rkon=1)
         rp.moddton = 0

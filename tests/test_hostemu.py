@@ -69,6 +69,21 @@ def test_adjoint_matches_replayed_differences(hostlib, case, align):
         assert close(r["amap"]), r["amap"]
 
 
+BC_CASES = ["bc:2d_symm_cont:1", "bc:2d_fixed_asymm:0", "bc:3d_cont_symm_fixed:1", "bc:3d_asymm_fixed_cont:1"]
+
+
+@pytest.mark.parametrize("case", BC_CASES)
+def test_boundary_types_forward_and_adjoint(hostlib, case):
+    """cont/fixed/symm/asymm tables: forward parity, and the adjoint's fold
+    (signs of asymm images, no cotangent through fixed ghosts)."""
+    r = run(hostlib, "fwd", case)
+    assert max(r["rel_l2"]) <= 1e-12, r
+    r = run(hostlib, "adj", case)
+    for f, pair in enumerate(r["fields"]):
+        assert close(pair), (f, pair)
+    assert close(r["amap"]), r["amap"]
+
+
 @pytest.mark.parametrize("group", ["flux", "eterm", "grav", "hrho", "he", "hmom", "hb"])
 @pytest.mark.parametrize("case", ["s2r", "s3r"])
 def test_adjoint_term_groups(hostlib, case, group):
