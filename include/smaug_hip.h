@@ -18,7 +18,11 @@
  *     the static background;
  *   - boundary codes (reference include/iotypes.h:144):
  *       0 periodic, 1 mpi, 2 mpiperiod, 3 cont, 4 contcd4, 5 fixed,
- *       6 symm, 7 asymm.  This build implements 0 and 4.
+ *       6 symm, 7 asymm, per field and direction (boundtype[f][d][0]).
+ *       This build implements 0 (2D), 3, 4, 5, 6 and 7; a fixed ghost keeps
+ *       the value of the initial state (or of the state last loaded), and
+ *       the directions are applied in the order 0, 1, 2 as the reference's
+ *       boundary passes do.
  *
  * Reference interfaces replaced (file:line in mikeg64/smaug):
  *   smaug_create      <- cuinit (src/init.cu:2222) + initgrid (src/init.cu:2502)

@@ -41,20 +41,15 @@ __device__ inline double dx_at(const Geo& G, const AdjArgs& A, int d, const int*
   return A.dx1 ? A.dx1[axis_off(G, d) + y[d]] : A.delx[(long long)d * G.N + gix(G, y[0], y[1], y[2])];
 }
 
-// nu_{i+1/2}, nu_{i-1/2} of q along d at interior cell y, from the same
-// 7-value window (with the halo substitutions) the stage kernel stages
 template <int ND>
-__device__ inline void nu_at(const Geo& G, const double* __restrict__ qa, const int* y, int d, double dxk,
-                             double& nr, double& nl) {
-  double Q[7];
-#pragma unroll
-  for (int o = 0; o < 7; o++) {
-    int z[3] = {y[0], y[1], y[2]};
-    z[d] += o - 3;
-    Q[o] = qval<ND, double>(G, qa, z);
-  }
-  nu7(Q, dxk, nr, nl);
-}
+struct Nb;
+
+// nu_{i+-1/2} at the cells x + k e_d, k = -1, 0, 1, that are interior
+// (ok[k+1]), from one 9-value window of q along d (the three 7-value
+// windows overlap); r = 1/delx along d at those cells
+template <int ND>
+__device__ inline void nu3_at(const Geo& G, const AdjArgs& A, const double* __restrict__ qa, const int* x, int d,
+                              double scale, bool* ok, double* nr, double* nl, double* r);
 
 template <int ND>
 struct Nb {  // cell y = x + k e_d (and optionally + k2 e_d2)
@@ -73,6 +68,28 @@ struct Nb {  // cell y = x + k e_d (and optionally + k2 e_d2)
     y[d2] += k2;
   }
 };
+
+template <int ND>
+__device__ inline void nu3_at(const Geo& G, const AdjArgs& A, const double* __restrict__ qa, const int* x, int d,
+                              double scale, bool* ok, double* nr, double* nl, double* r) {
+  double Q[9];
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    int z[3] = {x[0], x[1], x[2]};
+    z[d] += o - 4;
+    Q[o] = qval<ND, double>(G, qa, z);
+  }
+#pragma unroll
+  for (int k = -1; k <= 1; k++) {
+    Nb<ND> c(x, d, k);
+    ok[k + 1] = interior_c<ND>(G, c.y);
+    nr[k + 1] = nl[k + 1] = r[k + 1] = 0.0;
+    if (ok[k + 1]) {
+      nu7(&Q[k + 1], dx_at(G, A, d, c.y) * scale, nr[k + 1], nl[k + 1]);
+      r[k + 1] = rdx_at(G, A, d, c.y);
+    }
+  }
+}
 
 // (dF^d/du)^T Gf at one cell, added to lam.  F^d as in fluxes2:
 //   F_rho  = v_d rho + rhob v_d                       (= m_d)
@@ -221,8 +238,16 @@ __global__ __launch_bounds__(256) void k_adj_mu(Geo G, AdjMuArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// k_adj_rhs: T(p) = (d acc / d win)^T mu, one thread per cell p.
-template <int ND>
+// k_adj_rhs: T(p) = (d acc / d win)^T mu, one thread per cell p, in several
+// launches that each keep a modest register footprint:
+//   PART 0          fluxes (T = ...)
+//   PART 1          energy terms and gravity (T += ...)
+//   PART 4, FSEL 0  rho hyperdiffusion, FSEL 1 e hyperdiffusion (T += ...)
+//   PART 2, FSEL f  momentum hyperdiffusion of v_f (T += ...)
+//   PART 3, FSEL f  magnetic hyperdiffusion of b_f (T += ...)
+// Each part chains its own cotangents of v and q_e back to (rho, m, e, b) at
+// p, which is exact because the chain is linear.
+template <int ND, int PART, int FSEL>
 __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   constexpr int NP = 2 + 2 * ND;
   const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -240,8 +265,9 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   for (int d = 0; d < ND; d++) ng += (x[d] < 2 || x[d] > G.n[d] - 3) ? 1 : 0;
   // cells with two ghost coordinates are read by no stencil of an interior cell
   if (ng > 1) {
+    if (PART == 0)
 #pragma unroll
-    for (int f = 0; f < NP; f++) A.out[(long long)f * N + p] = 0.0;
+      for (int f = 0; f < NP; f++) A.out[(long long)f * N + p] = 0.0;
     return;
   }
   const double* __restrict__ w = A.win;
@@ -265,7 +291,7 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   // ---------------------------------------------------------- fluxes
   // acc_f(c) -= w_d(c) (8F(c+s) - 8F(c-s) + F(c-2s) - F(c+2s)),
   // w_d(c) = [c interior][no grad3dn zeroing] rdx_d(c)/12
-  if (T & SMAUG_T_FLUX) {
+  if (PART == 0 && (T & SMAUG_T_FLUX)) {
     double u[NP], bgc[2 + ND];
 #pragma unroll
     for (int f = 0; f < NP; f++) u[f] = w[(long long)f * N + p];
@@ -293,7 +319,7 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   // ---------------------------------------------------------- background energy terms
   // acc_e(c) += sum_{f,i} k_fi(c) D_i[v_f](c), k_fi = bb_i bb_f - [i=f] p_tb,
   // D_i the masked 4th-order difference (addenergyterms_cd2)
-  if (T & SMAUG_T_ETERM) {
+  if (PART == 1 && (T & SMAUG_T_ETERM)) {
     const int ks[4] = {1, -1, -2, 2};
     const double ak[4] = {8.0, -8.0, 1.0, -1.0};
 #pragma unroll
@@ -328,7 +354,7 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
 
   // ---------------------------------------------------------- gravity
   // acc_m_d += g_d rho;  acc_e += rho g_d m_d / rho_t
-  if ((T & SMAUG_T_GRAV) && inp) {
+  if (PART == 1 && (T & SMAUG_T_GRAV) && inp) {
     const double me = mu[(long long)fE(ND) * N + p];
     const double rho = w[p], rhob = bg[(long long)bRHOB() * N + p];
 #pragma unroll
@@ -342,9 +368,10 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   // ---------------------------------------------------------- rho and e hyperdiffusion
   // acc(c) += (nu_r z (q(c+s) - q(c)) - nu_l z (q(c) - q(c-s))) rdx(c)^2
   // gathered at p: mu(p-s) A_r(p-s) - mu(p)(A_r + A_l)(p) + mu(p+s) A_l(p+s)
-  if (G.hyper) {
+  if (PART == 4 && G.hyper) {
 #pragma unroll
     for (int which = 0; which < 2; which++) {
+      if (which != FSEL) continue;
       if (which == 0 && !(T & SMAUG_T_HRHO)) continue;
       if (which == 1 && !(T & SMAUG_T_HE)) continue;
       const int fld = which == 0 ? fRHO() : fE(ND);
@@ -353,17 +380,16 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
       double acc = 0.0;
 #pragma unroll
       for (int d = 0; d < ND; d++) {
+        bool ok[3];
+        double nr[3], nl[3], r[3];
+        nu3_at<ND>(G, A, qa, x, d, cm[d] * G.chyp[fld], ok, nr, nl, r);
 #pragma unroll
         for (int k = -1; k <= 1; k++) {
           Nb<ND> c(x, d, k);
-          if (!interior_c<ND>(G, c.y) || z1(G, zi, d, c.y[d])) continue;
+          if (!ok[k + 1] || z1(G, zi, d, c.y[d])) continue;
           const double m = mu[(long long)fld * N + at(c.y)];
-          if (m == 0.0) continue;
-          double nr, nl;
-          nu_at<ND>(G, qa, c.y, d, dx_at(G, A, d, c.y) * cm[d] * G.chyp[fld], nr, nl);
-          const double r = rdx_at(G, A, d, c.y);
-          const double coef = k == -1 ? nr : (k == 1 ? nl : -(nr + nl));
-          acc += m * coef * r * r;
+          const double coef = k == -1 ? nr[0] : (k == 1 ? nl[2] : -(nr[1] + nl[1]));
+          acc += m * coef * r[k + 1] * r[k + 1];
         }
       }
       if (which == 0) lam[fRHO()] += acc;
@@ -372,9 +398,10 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   }
 
   // ---------------------------------------------------------- momentum hyperdiffusion (q = v_f)
-  if (G.hyper && (T & SMAUG_T_HMOM)) {
+  if (PART == 2 && G.hyper && (T & SMAUG_T_HMOM)) {
 #pragma unroll
     for (int f = 0; f < ND; f++) {
+      if (f != FSEL) continue;
       const double* qa = A.prim + (long long)pV(f) * N;
       const double Kc = G.chyp[fMOM(f)];
 #pragma unroll
@@ -383,17 +410,7 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
         // nu, rdx and the masks at c = p + k e_d, k = -1, 0, 1
         double nr[3], nl[3], r[3];
         bool ok[3];
-#pragma unroll
-        for (int k = -1; k <= 1; k++) {
-          Nb<ND> c(x, d, k);
-          ok[k + 1] = interior_c<ND>(G, c.y);
-          nr[k + 1] = nl[k + 1] = 0.0;
-          r[k + 1] = 0.0;
-          if (ok[k + 1]) {
-            nu_at<ND>(G, qa, c.y, d, dx_at(G, A, d, c.y) * cm[d] * Kc, nr[k + 1], nl[k + 1]);
-            r[k + 1] = rdx_at(G, A, d, c.y);
-          }
-        }
+        nu3_at<ND>(G, A, qa, x, d, cm[d] * Kc, ok, nr, nl, r);
         // diagonal (src/hyperdifmomsource1.cu): with P = kap nu z rdx^2/2,
         //   acc_m_f(c) += P_r rr (v(c+s) - v(c)) - P_l rl (v(c) - v(c-s))
         //   acc_e(c)   += P_r mr (v(c+s) - v(c)) - P_l ml (v(c) - v(c-s))
@@ -476,9 +493,10 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
   }
 
   // ---------------------------------------------------------- magnetic hyperdiffusion (q = b_f)
-  if (G.hyper && (T & SMAUG_T_HB)) {
+  if (PART == 3 && G.hyper && (T & SMAUG_T_HB)) {
 #pragma unroll
     for (int f = 0; f < ND; f++) {
+      if (f != FSEL) continue;
       const double* qa = w + (long long)fB(ND, f) * N;
       const double Kc = G.chyp[fB(ND, f)];
       const double bfp = qa[p];
@@ -487,17 +505,7 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
         if (d == f) continue;
         double nr[3], nl[3], r[3];
         bool ok[3];
-#pragma unroll
-        for (int k = -1; k <= 1; k++) {
-          Nb<ND> c(x, d, k);
-          ok[k + 1] = interior_c<ND>(G, c.y);
-          nr[k + 1] = nl[k + 1] = 0.0;
-          r[k + 1] = 0.0;
-          if (ok[k + 1]) {
-            nu_at<ND>(G, qa, c.y, d, dx_at(G, A, d, c.y) * cm[d] * Kc, nr[k + 1], nl[k + 1]);
-            r[k + 1] = rdx_at(G, A, d, c.y);
-          }
-        }
+        nu3_at<ND>(G, A, qa, x, d, cm[d] * Kc, ok, nr, nl, r);
         // diagonal (src/hyperdifbsource1.cu), Q = nu z rdx^2:
         //   acc_b_f(c) += Q_r (b(c+s) - b(c)) - Q_l (b(c) - b(c-s))
         //   acc_e(c)   += Q_r (b(c+s)^2 - b(c)^2)/2 - Q_l (b(c)^2 - b(c-s)^2)/2
@@ -570,7 +578,10 @@ __global__ __launch_bounds__(256) void k_adj_rhs(Geo G, AdjArgs A) {
     lam[fRHO()] += rtbar;
   }
 #pragma unroll
-  for (int f = 0; f < NP; f++) A.out[(long long)f * N + p] = lam[f];
+  for (int f = 0; f < NP; f++) {
+    const long long q = (long long)f * N + p;
+    A.out[q] = PART == 0 ? lam[f] : A.out[q] + lam[f];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -632,11 +643,30 @@ void launch_adj_mu(const Geo& G, hipStream_t st, const AdjMuArgs& A) {
     hipLaunchKernelGGL(k_adj_mu<2>, dim3(nblk(G.N)), dim3(256), 0, st, G, A);
 }
 
+template <int ND>
+static void launch_adj_rhs_t(const Geo& G, hipStream_t st, const AdjArgs& A) {
+  const dim3 g(nblk(G.N)), b(256);
+  hipLaunchKernelGGL((k_adj_rhs<ND, 0, 0>), g, b, 0, st, G, A);
+  hipLaunchKernelGGL((k_adj_rhs<ND, 1, 0>), g, b, 0, st, G, A);
+  if (G.hyper && (G.terms & SMAUG_T_HRHO)) hipLaunchKernelGGL((k_adj_rhs<ND, 4, 0>), g, b, 0, st, G, A);
+  if (G.hyper && (G.terms & SMAUG_T_HE)) hipLaunchKernelGGL((k_adj_rhs<ND, 4, 1>), g, b, 0, st, G, A);
+  if (G.hyper && (G.terms & SMAUG_T_HMOM)) {
+    hipLaunchKernelGGL((k_adj_rhs<ND, 2, 0>), g, b, 0, st, G, A);
+    hipLaunchKernelGGL((k_adj_rhs<ND, 2, 1>), g, b, 0, st, G, A);
+    if (ND == 3) hipLaunchKernelGGL((k_adj_rhs<ND, 2, 2>), g, b, 0, st, G, A);
+  }
+  if (G.hyper && (G.terms & SMAUG_T_HB)) {
+    hipLaunchKernelGGL((k_adj_rhs<ND, 3, 0>), g, b, 0, st, G, A);
+    hipLaunchKernelGGL((k_adj_rhs<ND, 3, 1>), g, b, 0, st, G, A);
+    if (ND == 3) hipLaunchKernelGGL((k_adj_rhs<ND, 3, 2>), g, b, 0, st, G, A);
+  }
+}
+
 void launch_adj_rhs(const Geo& G, hipStream_t st, const AdjArgs& A) {
   if (G.nd == 3)
-    hipLaunchKernelGGL(k_adj_rhs<3>, dim3(nblk(G.N)), dim3(256), 0, st, G, A);
+    launch_adj_rhs_t<3>(G, st, A);
   else
-    hipLaunchKernelGGL(k_adj_rhs<2>, dim3(nblk(G.N)), dim3(256), 0, st, G, A);
+    launch_adj_rhs_t<2>(G, st, A);
 }
 
 void launch_adj_fold(const Geo& G, hipStream_t st, const double* T, double* lam) {

@@ -92,15 +92,40 @@ __host__ __device__ inline int bc_src(int type, int x, int n, int nd) {
   }
 }
 
-// ghost coordinates whose value is copied from interior coordinate y
-// (at most four per dim); returns how many were written to out[]
-__host__ __device__ inline int bc_images(int type, int y, int n, int nd, int* out) {
+// ghost coordinates whose value is copied from interior coordinate y (at
+// most mmax of them; with n >= 8 there are at most two); returns how many
+// were written to out[].  The inverse of bc_src, written out per type.
+__host__ __device__ inline int bc_images(int type, int y, int n, int nd, int* out, int mmax = 4) {
   int m = 0;
+  auto put = [&](int g) {
+    if (m < mmax) out[m++] = g;
+  };
   if (type == BC_FIXED) return 0;
-  const int g[4] = {0, 1, n - 2, n - 1};
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-    if (bc_src(type, g[q], n, nd) == y) out[m++] = g[q];
+  if (type == BC_PERIODIC) {
+    if (y == n - 4) put(0);
+    if (y == n - 3) put(1);
+    if (y == 2) put(n - 2);
+    if (y == 3) put(n - 1);
+  } else if (type == BC_CONT) {
+    if (y == 2) {
+      put(0);
+      put(1);
+    }
+    if (y == n - 3) {
+      put(n - 2);
+      put(n - 1);
+    }
+  } else if (nd == 2 && (type == BC_SYMM || type == BC_ASYMM)) {
+    if (y == 3) put(0);
+    if (y == 2) put(1);
+    if (y == n - 4) put(n - 1);
+    if (y == n - 3) put(n - 2);
+  } else {  // contcd4; symm/asymm in 3D
+    if (y == 4) put(0);
+    if (y == 3) put(1);
+    if (y == n - 5) put(n - 1);
+    if (y == n - 4) put(n - 2);
+  }
   return m;
 }
 

@@ -498,29 +498,31 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   }
   double* __restrict__ wo = A.wout;
   // ghost images: every combination of this cell's images across the
-  // directions (a fixed direction has none), with the asymm signs
+  // directions (a fixed direction has none; with n >= 8 an interior
+  // coordinate feeds at most two ghosts per direction), with the asymm signs
 #pragma unroll
   for (int f = 0; f < NP; f++) {
     if (f > 0 && G.bc_uniform) break;
-    int li[3][5], ln[3] = {1, 1, 1};
-    double sgn[3] = {1.0, 1.0, 1.0};
+    int li[3][3], ln[3] = {1, 1, 1};
+    unsigned neg = 0;  // bit d: direction d flips the sign
 #pragma unroll
     for (int d = 0; d < 3; d++) li[d][0] = x[d];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      ln[d] = 1 + bc_images(G.bct[f][d], x[d], G.n[d], ND, &li[d][1]);
-      sgn[d] = bc_sign(G.bct[f][d]);
+      ln[d] = 1 + bc_images(G.bct[f][d], x[d], G.n[d], ND, &li[d][1], 2);
+      if (G.bct[f][d] == BC_ASYMM) neg |= 1u << d;
     }
     for (int a = 0; a < ln[0]; a++)
       for (int b2 = 0; b2 < ln[1]; b2++)
         for (int c2 = 0; c2 < ln[2]; c2++) {
           const long long cc = gix(G, li[0][a], li[1][b2], li[2][c2]);
-          const double sg = (a ? sgn[0] : 1.0) * (b2 ? sgn[1] : 1.0) * (c2 ? sgn[2] : 1.0);
+          const unsigned flip = ((a ? 1u : 0u) & neg) ^ ((b2 ? 2u : 0u) & neg) ^ ((c2 ? 4u : 0u) & neg);
+          const bool minus = __builtin_popcount(flip) & 1;
           if (G.bc_uniform) {
 #pragma unroll
-            for (int q = 0; q < NP; q++) wo[(long long)q * N + cc] = sg * out[q];
+            for (int q = 0; q < NP; q++) wo[(long long)q * N + cc] = minus ? -out[q] : out[q];
           } else {
-            wo[(long long)f * N + cc] = sg * out[f];
+            wo[(long long)f * N + cc] = minus ? -out[f] : out[f];
           }
         }
   }
