@@ -95,6 +95,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--mixed-check-steps", type=int, default=0,
+                    help="also run FP64 and mixed from the initial state for this many steps (2000 for the "
+                         "mixed-precision configuration) and report their per-variable differences")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 256 MiB -> 256 MiB elementwise copy first (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -155,9 +158,14 @@ def main(argv=None):
     msm_prim, msm_rhs, _ = engm.time_kernels(args.kernel_steps)
     engm.close()
     k = 2 + 2 * rp.ndim
-    num = np.sqrt(((statem[:k] - state64[:k]) ** 2).sum(axis=1))
-    den = np.sqrt((state64[:k] ** 2).sum(axis=1))
-    rel = (num / np.where(den == 0, 1.0, den)).tolist()
+
+    def diffs(a, b):
+        num = np.sqrt(((a[:k] - b[:k]) ** 2).sum(axis=1))
+        den = np.sqrt((b[:k] ** 2).sum(axis=1))
+        mx = np.abs(a[:k] - b[:k]).max(axis=1) / np.maximum(np.abs(b[:k]).max(axis=1), 1e-300)
+        return (num / np.where(den == 0, 1.0, den)).tolist(), mx.tolist()
+
+    rel, mxe = diffs(statem, state64)
     del state64, statem
     mixed = {
         "value": cells * args.steps / wallm,
@@ -167,8 +175,32 @@ def main(argv=None):
         "k_rhs_ms_per_launch": msm_rhs,
         "k_prim_ms_per_launch": msm_prim,
         "rel_l2_vs_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], rel)),
+        "max_err_over_max_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], mxe)),
         "steps_compared": args.warmup + args.steps,
     }
+    if args.mixed_check_steps > 0:
+        n = args.mixed_check_steps
+        w, pos = configs.stratified(rp, perturb=20261202)
+        states = []
+        for prec in (0, PREC_MIXED):
+            rq = configs.stratified_params(ni=(S, S, S), rkon=1)
+            rq.precision = prec
+            e = Engine(rq, w, pos)
+            for done in range(0, n, 250):
+                e.step(min(250, n - done))
+                e.sync()
+                print(f"[bench] long run precision={prec}: {min(done + 250, n)}/{n} steps",
+                      file=sys.stderr, flush=True)
+            states.append(e.state())
+            e.close()
+        del w, pos
+        rel_n, mx_n = diffs(states[1], states[0])
+        mixed["long_run"] = {
+            "steps": n,
+            "rel_l2_vs_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], rel_n)),
+            "max_err_over_max_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], mx_n)),
+        }
+        del states
     rhs_bytes = RHS_BYTES_PER_CELL * cells
     achieved = rhs_bytes / (ms_rhs * 1e-3) / 1e9
     roof = {
