@@ -181,24 +181,34 @@ def main(argv=None):
     if args.mixed_check_steps > 0:
         n = args.mixed_check_steps
         w, pos = configs.stratified(rp, perturb=20261202)
+        # third run: FP64 from the initial state with every evolved value
+        # moved by one ulp -- the scheme's own sensitivity to roundoff, the
+        # floor under any comparison of two operation orders after n steps
+        w_ulp = w.copy()
+        rng = np.random.default_rng(1)
+        w_ulp[:k] = np.nextafter(w_ulp[:k], np.where(rng.random(w_ulp[:k].shape) < 0.5, -np.inf, np.inf))
         states = []
-        for prec in (0, PREC_MIXED):
+        for label, prec, w_init in (("fp64", 0, w), ("mixed", PREC_MIXED, w), ("fp64 one-ulp", 0, w_ulp)):
             rq = configs.stratified_params(ni=(S, S, S), rkon=1)
             rq.precision = prec
-            e = Engine(rq, w, pos)
+            e = Engine(rq, w_init, pos)
             for done in range(0, n, 250):
                 e.step(min(250, n - done))
                 e.sync()
-                print(f"[bench] long run precision={prec}: {min(done + 250, n)}/{n} steps",
+                print(f"[bench] long run {label}: {min(done + 250, n)}/{n} steps",
                       file=sys.stderr, flush=True)
             states.append(e.state())
             e.close()
-        del w, pos
+        del w, pos, w_ulp
         rel_n, mx_n = diffs(states[1], states[0])
+        rel_u, mx_u = diffs(states[2], states[0])
+        names = field_names(rp.ndim)[:k]
         mixed["long_run"] = {
             "steps": n,
-            "rel_l2_vs_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], rel_n)),
-            "max_err_over_max_fp64_per_variable": dict(zip(field_names(rp.ndim)[:k], mx_n)),
+            "rel_l2_vs_fp64_per_variable": dict(zip(names, rel_n)),
+            "max_err_over_max_fp64_per_variable": dict(zip(names, mx_n)),
+            "fp64_one_ulp_spread_rel_l2_per_variable": dict(zip(names, rel_u)),
+            "fp64_one_ulp_spread_max_err_per_variable": dict(zip(names, mx_u)),
         }
         del states
     rhs_bytes = RHS_BYTES_PER_CELL * cells
