@@ -59,7 +59,13 @@ struct Lay {
                                     : imax((TX + 4) * TY, TX * (TY + 4));
   static constexpr int HYP = QN + 4 * RN;
   static constexpr int FLX = NP * FN;
-  static constexpr int SMEM = HYP > FLX ? HYP : FLX;
+  // cell box: tile + 1-cell halo, FP64; box 0 rho_t, boxes 1..ND the
+  // components of the phase's vector (m in the momentum phases, b in the
+  // field phases); it sits ahead of the area the phases share
+  static constexpr int PX = TX + 2, PY = TY + 2, PZ = ND == 3 ? TZ + 2 : 1;
+  static constexpr int PN = PX * PY * PZ;
+  static constexpr int CB = (ND + 1) * PN;
+  static constexpr int SMEM = CB + (HYP > FLX ? HYP : FLX);
 };
 
 template <int ND, typename SH>
@@ -84,6 +90,24 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
     const int a = l % L::QX, b = (l / L::QX) % L::QY, c = l / (L::QX * L::QY);
     int x[3] = {o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 + c : 0};
     qb[l] = qval<ND, SH>(G, qa, x);
+  }
+}
+
+// cell box of the tile plus one cell: rho_t = rho + rho_b (when with_rho)
+// and the ND components f0.. of the state, as the FP64 values the terms use
+// (cells outside the grid hold 0 and are never read)
+template <int ND, typename SH>
+__device__ inline void stage_cells(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
+                                   double* __restrict__ CBX, const int* o, bool with_rho, int f0) {
+  using L = Lay<ND>;
+  for (int l = threadIdx.x; l < L::PN; l += 256) {
+    const int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
+    const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
+    const bool in = in_grid<ND>(G, x);
+    const long long g = in ? gix(G, x[0], x[1], x[2]) : 0;
+    if (with_rho) CBX[l] = in ? (double)ws[g] + bg[(long long)bRHOB() * G.N + g] : 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) CBX[(1 + d) * L::PN + l] = in ? (double)ws[(long long)(f0 + d) * G.N + g] : 0.0;
   }
 }
 
@@ -133,9 +157,19 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
     }
   }
 
+  double* const CBX = sm;  // cell box (tile + 1)
+  const int pc = (lc[0] + 1) + L::PX * ((lc[1] + 1) + L::PY * (ND == 3 ? lc[2] + 1 : 0));
+  const int ps[3] = {1, L::PX, L::PX * L::PY};
+  const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
+  const bool want_eterm = (T & SMAUG_T_ETERM) != 0;
+  // rho_t and m on the cell box for the momentum phases; its loads go out
+  // with the first flux direction's and are visible after that phase's
+  // barrier (or the first momentum box's)
+  if (want_mom) stage_cells<ND, SH>(G, ws, bg, CBX, o, true, fMOM(0));
+
   // ------------------------------------------------------------ fluxes
   if (T & SMAUG_T_FLUX) {
-    double* FB = sm;
+    double* FB = sm + L::CB;
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       const int ex[3] = {d == 0 ? 2 : 0, d == 1 ? 2 : 0, d == 2 ? 2 : 0};
@@ -173,16 +207,13 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   const double rt_c = rho_c + rhob_c;
 
   // ------------------------------------------------------------ hyperdiffusion + energy terms
-  SH* qb = reinterpret_cast<SH*>(sm);                                        // q box
-  double* NR = sm + (L::QN * sizeof(SH) + sizeof(double) - 1) / sizeof(double);  // region arrays
+  SH* qb = reinterpret_cast<SH*>(sm + L::CB);                                        // q box
+  double* NR = sm + L::CB + (L::QN * sizeof(SH) + sizeof(double) - 1) / sizeof(double);  // region arrays
   double* RG[4] = {NR, NR + L::RN, NR + 2 * L::RN, NR + 3 * L::RN};  // GA/GB of the two cross directions
-  // tile-cell index in the q box and grid strides
+  // tile-cell index in the q box and its strides
   const int qc = (lc[0] + 3) + L::QX * ((lc[1] + 3) + L::QY * (ND == 3 ? lc[2] + 3 : 0));
   const int qs[3] = {1, L::QX, L::QX * L::QY};
-  const long long gs[3] = {1, G.n[0], (long long)G.n[0] * G.n[1]};
 
-  const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
-  const bool want_eterm = (T & SMAUG_T_ETERM) != 0;
   double cm[ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) cm[d] = __longlong_as_double((long long)A.cmax_bits[d]);
@@ -276,7 +307,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
                   z1(G, zTMP4, d, x[d]) ? 0.0 : ((double)qb[qc + qs[d]] - (double)qb[qc - qs[d]]) * rdx[d] * 0.5;
               const double t6 = t5 * (nur[d] + nul[d]) * 0.25;
               g7 = rt_c * t6;
-              g8 = t6 * (double)ws[(long long)fMOM(d) * N + c];
+              g8 = t6 * CBX[(1 + d) * L::PN + pc];
             }
             RG[2 * k][ri] = g7;
             RG[2 * k + 1][ri] = g8;
@@ -297,7 +328,8 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
             int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
             const bool in = interior_c<ND>(G, xr);
             const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
-            const double rt = in ? (double)ws[gq] + bg[(long long)bRHOB() * N + gq] : 0.0;
+            const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
+            const double rt = in ? CBX[pi] : 0.0;
             const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
 #pragma unroll
             for (int d = 0; d < ND; d++) {
@@ -314,7 +346,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
                     z1(G, zTMP4, d, xr[d]) ? 0.0 : ((double)qb[qi + qs[d]] - (double)qb[qi - qs[d]]) * rdxr * 0.5;
                 const double t6 = t5 * (nr + nl) * 0.25;
                 g7 = rt * t6;
-                g8 = t6 * (double)ws[(long long)fMOM(d) * N + gq];
+                g8 = t6 * CBX[(1 + d) * L::PN + pi];
               }
               RG[2 * k][l] = g7;
               RG[2 * k + 1][l] = g8;
@@ -323,20 +355,19 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         }
         __syncthreads();
         if (valid) {
-          const double* rb = bg + (long long)bRHOB() * N;
-          const SH* mf = ws + (long long)fMOM(f) * N;
-          const double mc = mf[c];
+          const double* mf = CBX + (1 + f) * L::PN + pc;
+          const double mc = mf[0];
 #pragma unroll
           for (int d = 0; d < ND; d++) {
             const int s = qs[d];
-            const long long sg = gs[d];
+            const int sp = ps[d];
             const bool z = z1(G, zTMP4, d, x[d]);
             const double q0 = (double)qb[qc];
             const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[d];
             const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[d];
-            const double rtp = (double)ws[c + sg] + rb[c + sg], rtm = (double)ws[c - sg] + rb[c - sg];
+            const double rtp = CBX[pc + sp], rtm = CBX[pc - sp];
             const double rr = (rt_c + rtp) * 0.5, rl = (rt_c + rtm) * 0.5;
-            const double mr = (mc + (double)mf[c + sg]) * 0.5, ml = (mc + (double)mf[c - sg]) * 0.5;
+            const double mr = (mc + mf[sp]) * 0.5, ml = (mc + mf[-sp]) * 0.5;
             const double dmv = (rr * nur[d] * gr - rl * nul[d] * gl) * (rdx[d] * 0.5);
             const double dev = (mr * nur[d] * gr - ml * nul[d] * gl) * (rdx[d] * 0.5);
             acc[fMOM(f)] += dmv;
@@ -364,6 +395,9 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   // src/hyperdifbsourcene1.cu) and diagonal terms (sb = +1,
   // src/hyperdifbsource1.cu), one region pass per f for both d != f
   if (G.hyper && (T & SMAUG_T_HB)) {
+    // b on the cell box (the momentum phases are done with it: each ended
+    // with a barrier)
+    stage_cells<ND, SH>(G, ws, bg, CBX, o, false, fB(ND, 0));
 #pragma unroll
     for (int f = 0; f < ND; f++) {
       stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o);
@@ -386,7 +420,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
               z1(G, zTMP1, d, x[d]) ? 0.0 : ((double)qb[qc + qs[d]] - (double)qb[qc - qs[d]]) * rdx[d] * 0.5;
           const double t3 = t2 * (nul[d] + nur[d]) * 0.5;
           g3 = t3;
-          g5 = t3 * (double)ws[(long long)fB(ND, d) * N + c];
+          g5 = t3 * CBX[(1 + d) * L::PN + pc];
         }
         RG[2 * k][ri] = g3;
         RG[2 * k + 1][ri] = g5;
@@ -406,6 +440,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
           int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
           const bool in = interior_c<ND>(G, xr);
           const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
+          const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
           const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
 #pragma unroll
           for (int d = 0; d < ND; d++) {
@@ -422,7 +457,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
                   z1(G, zTMP1, d, xr[d]) ? 0.0 : ((double)qb[qi + qs[d]] - (double)qb[qi - qs[d]]) * rdxr * 0.5;
               const double t3 = t2 * (nl + nr) * 0.5;
               g3 = t3;
-              g5 = t3 * (double)ws[(long long)fB(ND, d) * N + gq];
+              g5 = t3 * CBX[(1 + d) * L::PN + pi];
             }
             RG[2 * k][l] = g3;
             RG[2 * k + 1][l] = g5;
