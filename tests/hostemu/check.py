@@ -53,28 +53,55 @@ def make_case(name: str, terms: int = P.TERMS_ALL):
     return rp, w, pos, amap, il, nsteps
 
 
+def caller_buffer(n: int, fill: float = 0.0) -> np.ndarray:
+    """n doubles of caller memory for the C ABI, allocated by the host
+    build's own hipMalloc, so the buffer sits against a guard page like the
+    engine's buffers (EMU_ALIGN picks the leading or trailing one) and an
+    access past either end of it ends the check."""
+    import ctypes
+
+    from smaug_amd import engine as E
+
+    lib = E.load()
+    alloc = lib._Z9hipMallocPPvm
+    alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    alloc.restype = ctypes.c_int
+    ptr = ctypes.c_void_p()
+    assert alloc(ctypes.byref(ptr), 8 * n) == 0 and ptr.value
+    a = np.ctypeslib.as_array((ctypes.c_double * n).from_address(ptr.value))
+    a[:] = fill
+    return a
+
+
 def engine_adjoint(rp, w0, pos, amap, nsteps, i_line):
-    """Taped steps and adjoint steps through the C ABI, host buffers as the
-    "device" pointers (the host build's memory is host memory)."""
+    """Taped steps and adjoint steps through the C ABI; the caller's buffers
+    (tapes, end state, lambda, map gradient, the driver map) are guarded
+    host-build allocations standing in for device memory."""
     nd = rp.ndim
     k = P.npert(nd)
     e = Engine(rp, w0, pos)
     if amap is not None:
-        e.set_driver_map(amap)
+        am = caller_buffer(amap.size)
+        am[:] = amap
+        e.set_driver_map(device_ptr=am.ctypes.data)
     F = k * rp.ncells
     T = e.tape_doubles()
-    tapes = np.full((nsteps, T), np.nan)
+    tapes = [caller_buffer(T, np.nan) for _ in range(nsteps)]
     for n in range(nsteps):
         e.step_taped(tapes[n].ctypes.data)
-    wend = np.zeros(F)
+    wend = caller_buffer(F)
     e.save_pert(wend.ctypes.data)
-    lam = np.zeros(F)
-    gmap = np.zeros(rp.ncells // rp.n[0])
+    lam = caller_buffer(F)
+    gmap = caller_buffer(rp.ncells // rp.n[0])
     shape = tuple(rp.n[:nd][::-1])
     rhob = w0[3 + 2 * nd].reshape(shape)
     inner = tuple(slice(2, rp.n[d] - 2) for d in range(nd - 1, 0, -1)) + (i_line,)
     for n in reversed(range(nsteps)):
-        wn = wend if n == nsteps - 1 else tapes[n + 1, :F].copy()
+        if n == nsteps - 1:
+            wn = wend
+        else:
+            wn = caller_buffer(F)
+            wn[:] = tapes[n + 1][:F]
         v = wn.reshape((k,) + shape)
         rt = v[0][inner] + rhob[inner]
         vz = v[1][inner] / rt
