@@ -28,11 +28,12 @@ __device__ inline bool interior_c(const Geo& G, const int* x) {
   return true;
 }
 
-// q at grid coordinates x, one step past the grid allowed in one direction:
-// coordinate -1 reads coordinate 5 (left-chain halo, quirk Q8), coordinate n
-// reads the right-chain stray halo (quirk Q7).
-template <int ND, typename SH>
-__device__ inline SH qval(const Geo& G, const SH* __restrict__ qa, const int* x) {
+// Cell whose q stands at grid coordinates x, one step past the grid allowed
+// in one direction: coordinate -1 reads coordinate 5 (left-chain halo, quirk
+// Q8), coordinate n reads the right-chain stray halo (quirk Q7); -1 = the
+// value is 0.  (Cell indices fit an int: a context holds 51 doubles per cell.)
+template <int ND>
+__device__ inline int qsrc(const Geo& G, const int* x) {
   int out = -1, nout = 0;
 #pragma unroll
   for (int d = 0; d < ND; d++)
@@ -40,22 +41,27 @@ __device__ inline SH qval(const Geo& G, const SH* __restrict__ qa, const int* x)
       out = d;
       nout++;
     }
-  if (nout == 0) return qa[gix(G, x[0], x[1], ND == 3 ? x[2] : 0)];
-  if (nout > 1) return (SH)0;
+  if (nout == 0) return (int)gix(G, x[0], x[1], ND == 3 ? x[2] : 0);
+  if (nout > 1) return -1;
   int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
   if (x[out] == -1) {
     y[out] = 5;
-    return qa[gix(G, y[0], y[1], y[2])];
+    return (int)gix(G, y[0], y[1], y[2]);
   }
   if (x[out] == G.n[out]) {
     int pc[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) pc[q] = y[q] + 1;
     pc[out] = G.n[out] + 1;
-    const long long h = halo_r_src(G, pc);
-    return h < 0 ? (SH)0 : qa[h];
+    return (int)halo_r_src(G, pc);
   }
-  return (SH)0;
+  return -1;
+}
+
+template <int ND, typename SH>
+__device__ inline SH qval(const Geo& G, const SH* __restrict__ qa, const int* x) {
+  const int h = qsrc<ND>(G, x);
+  return h < 0 ? (SH)0 : qa[h];
 }
 
 // nu_{i+1/2}, nu_{i-1/2} from the 7 values q(i-3..i+3) along one direction:
@@ -89,22 +95,35 @@ __device__ inline void nu_win(const SH* __restrict__ qb, int ci, int s, double d
 
 // fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,
 // src/centdiff2.cu:158-356)
+// the state of one cell that its fluxes need
+template <int ND>
+struct CellState {
+  double m[ND], b[ND], bb[ND], rho, rhob, e, eb;
+};
+
 template <int ND, typename SH>
-__device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
-                               int d, double* F) {
+__device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
+                                 CellState<ND>& S) {
+  const long long N = G.N;
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    S.m[q] = w[(long long)fMOM(q) * N + cc];
+    S.b[q] = w[(long long)fB(ND, q) * N + cc];
+    S.bb[q] = bg[(long long)bBB(q) * N + cc];
+  }
+  S.rho = w[cc];
+  S.rhob = bg[(long long)bRHOB() * N + cc];
+  S.e = w[(long long)fE(ND) * N + cc];
+  S.eb = bg[(long long)bEB() * N + cc];
+}
+
+template <int ND>
+__device__ inline void fluxes_of(const Geo& G, const CellState<ND>& S, int d, double* F) {
   // v = m/rho_t and p_t are recomputed from the cell's state (4 fewer
   // loads per flux cell than reading k_prim's copies); p_t follows
   // computept3, src/dervfields.cu:107-153
-  const long long N = G.N;
-  double m[ND], b[ND], bb[ND];
-#pragma unroll
-  for (int q = 0; q < ND; q++) {
-    m[q] = w[(long long)fMOM(q) * N + cc];
-    b[q] = w[(long long)fB(ND, q) * N + cc];
-    bb[q] = bg[(long long)bBB(q) * N + cc];
-  }
-  const double rho = w[cc], rhob = bg[(long long)bRHOB() * N + cc];
-  const double e = w[(long long)fE(ND) * N + cc], eb = bg[(long long)bEB() * N + cc];
+  const double *m = S.m, *b = S.b, *bb = S.bb;
+  const double rho = S.rho, rhob = S.rhob, e = S.e, eb = S.eb;
   const double rt = rho + rhob;
   double v[ND];
 #pragma unroll
@@ -141,6 +160,14 @@ __device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const dou
 #pragma unroll
   for (int f = 0; f < ND; f++)
     F[fB(ND, f)] = (f == d) ? 0.0 : vd * b[f] - (bd + bbd) * v[f] + bb[f] * vd;
+}
+
+template <int ND, typename SH>
+__device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
+                               int d, double* F) {
+  CellState<ND> S;
+  load_cell<ND, SH>(G, w, bg, cc, S);
+  fluxes_of<ND>(G, S, d, F);
 }
 
 }  // namespace smaug

@@ -86,10 +86,29 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
     }
     return;
   }
-  for (int l = threadIdx.x; l < L::QN; l += 256) {
-    const int a = l % L::QX, b = (l / L::QX) % L::QY, c = l / (L::QX * L::QY);
-    int x[3] = {o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 + c : 0};
-    qb[l] = qval<ND, SH>(G, qa, x);
+  // boxes crossing the grid edge (with the halo substitutions): in batches
+  // of NB cells per thread, every source first, then the NB loads in flight
+  // together, then the stores
+  constexpr int NB = ND == 3 ? 4 : 1;
+#pragma unroll 1
+  for (int l0 = threadIdx.x; l0 < L::QN; l0 += 256 * NB) {
+    int src[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      const int l = l0 + 256 * k;
+      src[k] = -1;
+      if (l < L::QN) {
+        const int a = l % L::QX, b = (l / L::QX) % L::QY, c = l / (L::QX * L::QY);
+        const int x[3] = {o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 + c : 0};
+        src[k] = qsrc<ND>(G, x);
+      }
+    }
+    SH v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) v[k] = src[k] >= 0 ? qa[src[k]] : (SH)0;
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+      if (l0 + 256 * k < L::QN) qb[l0 + 256 * k] = v[k];
   }
 }
 
@@ -100,14 +119,30 @@ template <int ND, typename SH>
 __device__ inline void stage_cells(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
                                    double* __restrict__ CBX, const int* o, bool with_rho, int f0) {
   using L = Lay<ND>;
-  for (int l = threadIdx.x; l < L::PN; l += 256) {
+  constexpr int NK = (L::PN + 255) / 256;
+  int g[NK];  // cell of each slot, -1 outside the grid (or past the box)
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    const int l = threadIdx.x + 256 * k;
     const int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
     const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
-    const bool in = in_grid<ND>(G, x);
-    const long long g = in ? gix(G, x[0], x[1], x[2]) : 0;
-    if (with_rho) CBX[l] = in ? (double)ws[g] + bg[(long long)bRHOB() * G.N + g] : 0.0;
+    g[k] = l < L::PN && in_grid<ND>(G, x) ? (int)gix(G, x[0], x[1], x[2]) : -1;
+  }
+  double v[NK][ND + 1];  // all loads in flight together, then the stores
 #pragma unroll
-    for (int d = 0; d < ND; d++) CBX[(1 + d) * L::PN + l] = in ? (double)ws[(long long)(f0 + d) * G.N + g] : 0.0;
+  for (int k = 0; k < NK; k++) {
+    v[k][0] = with_rho && g[k] >= 0 ? (double)ws[g[k]] + bg[(long long)bRHOB() * G.N + g[k]] : 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) v[k][1 + d] = g[k] >= 0 ? (double)ws[(long long)(f0 + d) * G.N + g[k]] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    const int l = threadIdx.x + 256 * k;
+    if (l < L::PN) {
+      if (with_rho) CBX[l] = v[k][0];
+#pragma unroll
+      for (int d = 0; d < ND; d++) CBX[(1 + d) * L::PN + l] = v[k][1 + d];
+    }
   }
 }
 
@@ -174,18 +209,34 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
     for (int d = 0; d < ND; d++) {
       const int ex[3] = {d == 0 ? 2 : 0, d == 1 ? 2 : 0, d == 2 ? 2 : 0};
       const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
-      for (int l = tid; l < rx * ry * rz; l += 256) {
+      // at most two region cells per thread: both cells' state loads go
+      // out together, then the fluxes are formed and stored
+      static_assert(L::FN <= 512, "two flux cells per thread");
+      const int R = rx * ry * rz;
+      CellState<ND> S[2];
+      bool in[2];
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const int l = tid + 256 * k;
         const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
-        int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
-        double F[NP];
-        if (in_grid<ND>(G, xr)) {
-          fluxes2<ND, SH>(G, ws, bg, gix(G, xr[0], xr[1], xr[2]), d, F);
-        } else {
+        const int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
+        in[k] = l < R && in_grid<ND>(G, xr);
+        if (in[k]) load_cell<ND, SH>(G, ws, bg, gix(G, xr[0], xr[1], xr[2]), S[k]);
+      }
 #pragma unroll
-          for (int f = 0; f < NP; f++) F[f] = 0.0;
+      for (int k = 0; k < 2; k++) {
+        const int l = tid + 256 * k;
+        if (l < R) {
+          double F[NP];
+          if (in[k]) {
+            fluxes_of<ND>(G, S[k], d, F);
+          } else {
+#pragma unroll
+            for (int f = 0; f < NP; f++) F[f] = 0.0;
+          }
+#pragma unroll
+          for (int f = 0; f < NP; f++) FB[f * L::FN + l] = F[f];
         }
-#pragma unroll
-        for (int f = 0; f < NP; f++) FB[f * L::FN + l] = F[f];
       }
       __syncthreads();
       if (valid && !z4(G, zFLUX(ND), d, x[d])) {
