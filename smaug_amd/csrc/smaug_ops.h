@@ -80,8 +80,20 @@ __device__ inline void nu7(const double* Q, double dxk, double& nur, double& nul
   const double t5 = fmax(fmax(D1[1], D1[2]), fmax(D1[3], D1[4]));
   const double m3r = fmax(fmax(t3, D3[4]), 0.0), m3l = fmax(fmax(t3, D3[1]), 0.0);
   const double m5r = fmax(fmax(t5, D1[5]), 0.0), m5l = fmax(fmax(t5, D1[0]), 0.0);
-  nur = m5r > 0 ? dxk * m3r / m5r : 0.0;
-  nul = m5l > 0 ? dxk * m3l / m5l : 0.0;
+  // one division for both: dxk / (m5r m5l), a zero maximum standing in as
+  // 1 (its nu is 0 anyway); each nu then carries two roundings more than
+  // dxk m3 / m5 (relative 1e-16).  Products outside the normal range take
+  // the two divisions.
+  const double a = m5r > 0 ? m5r : 1.0, b = m5l > 0 ? m5l : 1.0;
+  const double p = a * b;
+  if (p > 1e-290 && p < 1e290) {
+    const double q = dxk / p;
+    nur = m5r > 0 ? (m3r * b) * q : 0.0;
+    nul = m5l > 0 ? (m3l * a) * q : 0.0;
+  } else {
+    nur = m5r > 0 ? dxk * m3r / m5r : 0.0;
+    nul = m5l > 0 ? dxk * m3l / m5l : 0.0;
+  }
 }
 
 // nu from a window of q staged in LDS (centre index ci, stride s)
@@ -125,9 +137,12 @@ __device__ inline void fluxes_of(const Geo& G, const CellState<ND>& S, int d, do
   const double *m = S.m, *b = S.b, *bb = S.bb;
   const double rho = S.rho, rhob = S.rhob, e = S.e, eb = S.eb;
   const double rt = rho + rhob;
+  // one reciprocal for the ND + 1 quotients by rho_t (each then carries
+  // one rounding more than a division: relative 1e-16)
+  const double rrt = 1.0 / rt;
   double v[ND];
 #pragma unroll
-  for (int q = 0; q < ND; q++) v[q] = m[q] / rt;
+  for (int q = 0; q < ND; q++) v[q] = m[q] * rrt;
   double mm = m[0] * m[0], b2 = b[0] * b[0], bbb = b[0] * bb[0];
 #pragma unroll
   for (int q = 1; q < ND; q++) {
@@ -135,7 +150,7 @@ __device__ inline void fluxes_of(const Geo& G, const CellState<ND>& S, int d, do
     b2 = b2 + b[q] * b[q];
     bbb = bbb + b[q] * bb[q];
   }
-  const double pt = (G.gamma - 1.0) * (e - 0.5 * mm / rt) - (G.gamma - 2.0) * (bbb + 0.5 * b2);
+  const double pt = (G.gamma - 1.0) * (e - 0.5 * mm * rrt) - (G.gamma - 2.0) * (bbb + 0.5 * b2);
   double vd = v[0], bd = b[0], bbd = bb[0];
 #pragma unroll
   for (int q = 1; q < ND; q++)

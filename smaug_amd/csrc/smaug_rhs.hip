@@ -65,30 +65,71 @@ struct Lay {
   static constexpr int PX = TX + 2, PY = TY + 2, PZ = ND == 3 ? TZ + 2 : 1;
   static constexpr int PN = PX * PY * PZ;
   static constexpr int CB = (ND + 1) * PN;
-  static constexpr int SMEM = CB + (HYP > FLX ? HYP : FLX);
+  static constexpr int SHR = HYP > FLX ? HYP : FLX;  // area the phases share
+  // spacing table of a tensor-product grid: delx and 1/delx at the tile's
+  // coordinates along each axis (for the cross-term border cells)
+  static constexpr int DXT = TX + TY + TZ;
+  static constexpr int SMEM = CB + SHR + 2 * DXT;
+};
+
+// cell box of the tile plus one cell: rho_t = rho + rho_b (when with_rho)
+// and the ND components f0.. of the state, as the FP64 values the terms use
+// (cells outside the grid hold 0 and are never read).  Loads and stores are
+// separate steps so the loads can share a round trip with a q box's.
+template <int ND>
+struct CellBox {
+  double* CBX;  // null: no cell box with this q box
+  bool with_rho;
+  int f0;
 };
 
 template <int ND, typename SH>
-__device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o) {
+struct CellLoads {
+  static constexpr int NK = (Lay<ND>::PN + 255) / 256;
+  double v[NK][ND + 1];
+};
+
+template <int ND, typename SH>
+__device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
+                                  const CellBox<ND>& C, const int* o, CellLoads<ND, SH>& R) {
   using L = Lay<ND>;
-  // fast path: the halo box lies inside the grid, no substitutions needed;
-  // each thread owns one (a,b) column of the box and walks its planes
-  bool inside = o[0] - 3 >= 0 && o[1] - 3 >= 0 && o[0] + L::TX + 3 <= G.n[0] && o[1] + L::TY + 3 <= G.n[1];
-  if (ND == 3) inside = inside && o[2] - 3 >= 0 && o[2] + L::TZ + 3 <= G.n[2];
-  if (inside) {
-    constexpr int PL = L::QX * L::QY;
-    const long long zs = (long long)G.n[0] * G.n[1];
-    for (int l = threadIdx.x; l < PL; l += 256) {
-      const int a = l % L::QX, b = l / L::QX;
-      const SH* src = qa + gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
+  constexpr int NK = CellLoads<ND, SH>::NK;
+  int g[NK];  // cell of each slot, -1 outside the grid (or past the box)
 #pragma unroll
-      for (int cz = 0; cz < L::QZ; cz++) qb[cz * PL + l] = src[cz * zs];
-    }
-    return;
+  for (int k = 0; k < NK; k++) {
+    const int l = threadIdx.x + 256 * k;
+    const int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
+    const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
+    g[k] = l < L::PN && in_grid<ND>(G, x) ? (int)gix(G, x[0], x[1], x[2]) : -1;
   }
-  // boxes crossing the grid edge (with the halo substitutions): in batches
-  // of NB cells per thread, every source first, then the NB loads in flight
-  // together, then the stores
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    R.v[k][0] = C.with_rho && g[k] >= 0 ? (double)ws[g[k]] + bg[(long long)bRHOB() * G.N + g[k]] : 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) R.v[k][1 + d] = g[k] >= 0 ? (double)ws[(long long)(C.f0 + d) * G.N + g[k]] : 0.0;
+  }
+}
+
+template <int ND, typename SH>
+__device__ inline void cells_store(const CellBox<ND>& C, const CellLoads<ND, SH>& R) {
+  using L = Lay<ND>;
+#pragma unroll
+  for (int k = 0; k < CellLoads<ND, SH>::NK; k++) {
+    const int l = threadIdx.x + 256 * k;
+    if (l < L::PN) {
+      if (C.with_rho) C.CBX[l] = R.v[k][0];
+#pragma unroll
+      for (int d = 0; d < ND; d++) C.CBX[(1 + d) * L::PN + l] = R.v[k][1 + d];
+    }
+  }
+}
+
+// q box of a tile whose box crosses the grid edge (with the halo
+// substitutions): in batches of NB cells per thread, every source first,
+// then the NB loads in flight together, then the stores
+template <int ND, typename SH>
+__device__ inline void stage_q_edge(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o) {
+  using L = Lay<ND>;
   constexpr int NB = ND == 3 ? 4 : 1;
 #pragma unroll 1
   for (int l0 = threadIdx.x; l0 < L::QN; l0 += 256 * NB) {
@@ -112,38 +153,57 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
   }
 }
 
-// cell box of the tile plus one cell: rho_t = rho + rho_b (when with_rho)
-// and the ND components f0.. of the state, as the FP64 values the terms use
-// (cells outside the grid hold 0 and are never read)
+// q box (tile + 3-cell halo) of field qa into qb, with it the cell box C
+// when C.CBX is set and a second q box (qa2 into qb2) when qa2 is set: all
+// of their loads are issued before the first store
 template <int ND, typename SH>
-__device__ inline void stage_cells(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
-                                   double* __restrict__ CBX, const int* o, bool with_rho, int f0) {
+__device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o,
+                               const SH* __restrict__ ws = nullptr, const double* __restrict__ bg = nullptr,
+                               CellBox<ND> C = CellBox<ND>{nullptr, false, 0}, const SH* __restrict__ qa2 = nullptr,
+                               SH* __restrict__ qb2 = nullptr) {
   using L = Lay<ND>;
-  constexpr int NK = (L::PN + 255) / 256;
-  int g[NK];  // cell of each slot, -1 outside the grid (or past the box)
+  CellLoads<ND, SH> cl;
+  if (C.CBX) cells_load<ND, SH>(G, ws, bg, C, o, cl);
+  // fast path: the halo box lies inside the grid, no substitutions needed;
+  // each thread owns (a,b) columns of the box and walks their planes
+  bool inside = o[0] - 3 >= 0 && o[1] - 3 >= 0 && o[0] + L::TX + 3 <= G.n[0] && o[1] + L::TY + 3 <= G.n[1];
+  if (ND == 3) inside = inside && o[2] - 3 >= 0 && o[2] + L::TZ + 3 <= G.n[2];
+  if (inside) {
+    constexpr int PL = L::QX * L::QY, NKF = (PL + 255) / 256;
+    const long long zs = (long long)G.n[0] * G.n[1];
+    SH v[NKF][L::QZ], v2[NKF][L::QZ];
 #pragma unroll
-  for (int k = 0; k < NK; k++) {
-    const int l = threadIdx.x + 256 * k;
-    const int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
-    const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
-    g[k] = l < L::PN && in_grid<ND>(G, x) ? (int)gix(G, x[0], x[1], x[2]) : -1;
-  }
-  double v[NK][ND + 1];  // all loads in flight together, then the stores
+    for (int k = 0; k < NKF; k++) {
+      const int l = threadIdx.x + 256 * k;
+      if (l < PL) {
+        const int a = l % L::QX, b = l / L::QX;
+        const long long g0 = gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
 #pragma unroll
-  for (int k = 0; k < NK; k++) {
-    v[k][0] = with_rho && g[k] >= 0 ? (double)ws[g[k]] + bg[(long long)bRHOB() * G.N + g[k]] : 0.0;
+        for (int cz = 0; cz < L::QZ; cz++) v[k][cz] = qa[g0 + cz * zs];
+        if (qa2) {
 #pragma unroll
-    for (int d = 0; d < ND; d++) v[k][1 + d] = g[k] >= 0 ? (double)ws[(long long)(f0 + d) * G.N + g[k]] : 0.0;
-  }
-#pragma unroll
-  for (int k = 0; k < NK; k++) {
-    const int l = threadIdx.x + 256 * k;
-    if (l < L::PN) {
-      if (with_rho) CBX[l] = v[k][0];
-#pragma unroll
-      for (int d = 0; d < ND; d++) CBX[(1 + d) * L::PN + l] = v[k][1 + d];
+          for (int cz = 0; cz < L::QZ; cz++) v2[k][cz] = qa2[g0 + cz * zs];
+        }
+      }
     }
+    if (C.CBX) cells_store<ND, SH>(C, cl);
+#pragma unroll
+    for (int k = 0; k < NKF; k++) {
+      const int l = threadIdx.x + 256 * k;
+      if (l < PL) {
+#pragma unroll
+        for (int cz = 0; cz < L::QZ; cz++) qb[cz * PL + l] = v[k][cz];
+        if (qa2) {
+#pragma unroll
+          for (int cz = 0; cz < L::QZ; cz++) qb2[cz * PL + l] = v2[k][cz];
+        }
+      }
+    }
+    return;
   }
+  if (C.CBX) cells_store<ND, SH>(C, cl);
+  stage_q_edge<ND, SH>(G, qa, qb, o);
+  if (qa2) stage_q_edge<ND, SH>(G, qa2, qb2, o);
 }
 
 template <int ND, typename SH>
@@ -192,15 +252,21 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
     }
   }
 
+  double* const SDX = sm + L::CB + L::SHR;  // spacing table (tensor-product grids)
+  if (A.dx1 && tid < L::DXT) {  // visible after the first barrier
+    const int d = tid < TX ? 0 : (tid < TX + TY ? 1 : 2);
+    const int xx = o[d] + tid - (d == 0 ? 0 : (d == 1 ? TX : TX + TY));
+    const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
+    const bool ing = d < ND && xx < G.n[d];
+    SDX[tid] = ing ? A.dx1[od + xx] : 1.0;
+    SDX[L::DXT + tid] = ing ? A.rdx1[od + xx] : 1.0;
+  }
+
   double* const CBX = sm;  // cell box (tile + 1)
   const int pc = (lc[0] + 1) + L::PX * ((lc[1] + 1) + L::PY * (ND == 3 ? lc[2] + 1 : 0));
   const int ps[3] = {1, L::PX, L::PX * L::PY};
   const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
   const bool want_eterm = (T & SMAUG_T_ETERM) != 0;
-  // rho_t and m on the cell box for the momentum phases; its loads go out
-  // with the first flux direction's and are visible after that phase's
-  // barrier (or the first momentum box's)
-  if (want_mom) stage_cells<ND, SH>(G, ws, bg, CBX, o, true, fMOM(0));
 
   // ------------------------------------------------------------ fluxes
   if (T & SMAUG_T_FLUX) {
@@ -269,10 +335,17 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
 #pragma unroll
   for (int d = 0; d < ND; d++) cm[d] = __longlong_as_double((long long)A.cmax_bits[d]);
 
-  if (G.hyper && (T & SMAUG_T_HRHO)) {  // src/hyperdifrhosource1.cu
-    stage_q<ND, SH>(G, ws, qb, o);
+  // rho (src/hyperdifrhosource1.cu) and e_int (src/hyperdifesource1.cu):
+  // both boxes are staged in one round trip, the e_int box after the rho box
+  const bool h_rho = G.hyper && (T & SMAUG_T_HRHO), h_e = G.hyper && (T & SMAUG_T_HE);
+  static_assert(2 * L::QN * sizeof(SH) <= L::SHR * sizeof(double), "two q boxes in the shared area");
+  if (h_rho || h_e) {
+    const SH* qe = pr + (long long)pQE(ND) * N;
+    SH* qbe = h_rho ? qb + L::QN : qb;
+    stage_q<ND, SH>(G, h_rho ? ws : qe, qb, o, ws, bg, CellBox<ND>{nullptr, false, 0}, h_rho && h_e ? qe : nullptr,
+                    h_rho && h_e ? qbe : nullptr);
     __syncthreads();
-    if (valid) {
+    if (valid && h_rho) {
 #pragma unroll
       for (int dim = 0; dim < ND; dim++) {
         double nur, nul;
@@ -284,20 +357,15 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         acc[fRHO()] += (nur * gr - nul * gl) * rdx[dim];
       }
     }
-    __syncthreads();
-  }
-  if (G.hyper && (T & SMAUG_T_HE)) {  // src/hyperdifesource1.cu
-    stage_q<ND, SH>(G, pr + (long long)pQE(ND) * N, qb, o);
-    __syncthreads();
-    if (valid) {
+    if (valid && h_e) {
 #pragma unroll
       for (int dim = 0; dim < ND; dim++) {
         double nur, nul;
-        nu_win(qb, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fE(ND)], nur, nul);
+        nu_win(qbe, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fE(ND)], nur, nul);
         const bool z = z1(G, zTMP1, dim, x[dim]);
-        const double q0 = (double)qb[qc];
-        const double gr = z ? 0.0 : ((double)qb[qc + qs[dim]] - q0) * rdx[dim];
-        const double gl = z ? 0.0 : (q0 - (double)qb[qc - qs[dim]]) * rdx[dim];
+        const double q0 = (double)qbe[qc];
+        const double gr = z ? 0.0 : ((double)qbe[qc + qs[dim]] - q0) * rdx[dim];
+        const double gl = z ? 0.0 : (q0 - (double)qbe[qc - qs[dim]]) * rdx[dim];
         acc[fE(ND)] += (nur * gr - nul * gl) * rdx[dim];
       }
     }
@@ -319,7 +387,9 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   if (want_mom || want_eterm) {
 #pragma unroll
     for (int f = 0; f < ND; f++) {
-      stage_q<ND, SH>(G, pr + (long long)pV(f) * N, qb, o);
+      // the first box brings rho_t and m on the cell box with it
+      stage_q<ND, SH>(G, pr + (long long)pV(f) * N, qb, o, ws, bg,
+                      CellBox<ND>{f == 0 && want_mom ? CBX : nullptr, true, fMOM(0)});
       __syncthreads();
       if (want_eterm && valid) {  // addenergyterms_cd2 for dir f
         double gf[ND];
@@ -388,9 +458,10 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
               const int k = d < f ? d : d - 1;
               double g7 = 0, g8 = 0;
               if (in) {
-                const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
-                const double dxr = A.dx1 ? A.dx1[od + xr[d]] : A.delx[(long long)d * N + gq];
-                const double rdxr = A.dx1 ? A.rdx1[od + xr[d]] : A.rdelx[(long long)d * N + gq];
+                // d != f: the border cell's coordinate along d is a tile coordinate
+                const int ti = (d == 0 ? 0 : (d == 1 ? TX : TX + TY)) + rel[d];
+                const double dxr = A.dx1 ? SDX[ti] : A.delx[(long long)d * N + gq];
+                const double rdxr = A.dx1 ? SDX[L::DXT + ti] : A.rdelx[(long long)d * N + gq];
                 double nr, nl;
                 nu_win(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
                 const double t5 =
@@ -446,12 +517,12 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   // src/hyperdifbsourcene1.cu) and diagonal terms (sb = +1,
   // src/hyperdifbsource1.cu), one region pass per f for both d != f
   if (G.hyper && (T & SMAUG_T_HB)) {
-    // b on the cell box (the momentum phases are done with it: each ended
-    // with a barrier)
-    stage_cells<ND, SH>(G, ws, bg, CBX, o, false, fB(ND, 0));
 #pragma unroll
     for (int f = 0; f < ND; f++) {
-      stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o);
+      // the first box brings b on the cell box with it (the momentum
+      // phases are done with the cell box: each ended with a barrier)
+      stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o, ws, bg,
+                      CellBox<ND>{f == 0 ? CBX : nullptr, false, fB(ND, 0)});
       __syncthreads();
       const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
       const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1];
@@ -499,9 +570,10 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
             const int k = d < f ? d : d - 1;
             double g3 = 0, g5 = 0;
             if (in) {
-              const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
-              const double dxr = A.dx1 ? A.dx1[od + xr[d]] : A.delx[(long long)d * N + gq];
-              const double rdxr = A.dx1 ? A.rdx1[od + xr[d]] : A.rdelx[(long long)d * N + gq];
+              // d != f: the border cell's coordinate along d is a tile coordinate
+              const int ti = (d == 0 ? 0 : (d == 1 ? TX : TX + TY)) + rel[d];
+              const double dxr = A.dx1 ? SDX[ti] : A.delx[(long long)d * N + gq];
+              const double rdxr = A.dx1 ? SDX[L::DXT + ti] : A.rdelx[(long long)d * N + gq];
               double nr, nl;
               nu_win(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
               const double t2 =
