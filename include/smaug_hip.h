@@ -76,7 +76,7 @@ extern "C" {
 
 /* arithmetic of the evolved state */
 #define SMAUG_PREC_FP64 0
-#define SMAUG_PREC_MIXED 1        /* FP32 stencil shadows, FP64 master     */
+#define SMAUG_PREC_MIXED 1        /* FP32 stencils, FP64 state and update  */
 
 typedef struct smaug_params {
   int ndim;                 /* 2 or 3                                        */

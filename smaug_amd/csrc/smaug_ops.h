@@ -12,6 +12,13 @@ __device__ inline long long gix(const Geo& G, int a, int b, int c) {
   return ((long long)c * G.n[1] + b) * G.n[0] + a;
 }
 
+// |x| and max(a, b) in the arithmetic type of the call (FP64, or FP32 for
+// the mixed-precision stencils); max ignores a NaN operand
+__device__ inline double tabs(double x) { return fabs(x); }
+__device__ inline float tabs(float x) { return fabsf(x); }
+__device__ inline double tmax(double a, double b) { return fmax(a, b); }
+__device__ inline float tmax(float a, float b) { return fmaxf(a, b); }
+
 template <int ND>
 __device__ inline bool in_grid(const Geo& G, const int* x) {
 #pragma unroll
@@ -67,91 +74,97 @@ __device__ inline SH qval(const Geo& G, const SH* __restrict__ qa, const int* x)
 // nu_{i+1/2}, nu_{i-1/2} from the 7 values q(i-3..i+3) along one direction:
 // third differences at 4 interfaces, first differences at 6, maxima over 3
 // and 5 interfaces, nu = dxk * max3 / max5 (src/hyperdifvisc1r.cu:426-708,
-// src/hyperdifvisc1l.cu:723-853); dxk = delx * cmax * chyp.
-__device__ inline void nu7(const double* Q, double dxk, double& nur, double& nul) {
-  double D3[5], D1[6];
+// src/hyperdifvisc1l.cu:723-853); dxk = delx * cmax * chyp.  T is the
+// arithmetic type (double; float for the mixed-precision stencils).
+template <typename T>
+__device__ inline void nu7(const T* Q, T dxk, T& nur, T& nul) {
+  T D3[5], D1[6];
 #pragma unroll
-  for (int a = 1; a <= 4; a++) D3[a] = fabs(3.0 * (Q[a + 1] - Q[a]) - (Q[a + 2] - Q[a - 1]));
+  for (int a = 1; a <= 4; a++) D3[a] = tabs(T(3) * (Q[a + 1] - Q[a]) - (Q[a + 2] - Q[a - 1]));
 #pragma unroll
-  for (int a = 0; a <= 5; a++) D1[a] = fabs(Q[a + 1] - Q[a]);
+  for (int a = 0; a <= 5; a++) D1[a] = tabs(Q[a + 1] - Q[a]);
   // maxima over the right (i+1/2) and left (i-1/2) windows share their
-  // middle; fmax ignores NaN as the reference's "if (v > max)" does
-  const double t3 = fmax(D3[2], D3[3]);
-  const double t5 = fmax(fmax(D1[1], D1[2]), fmax(D1[3], D1[4]));
-  const double m3r = fmax(fmax(t3, D3[4]), 0.0), m3l = fmax(fmax(t3, D3[1]), 0.0);
-  const double m5r = fmax(fmax(t5, D1[5]), 0.0), m5l = fmax(fmax(t5, D1[0]), 0.0);
-  // one division for both: dxk / (m5r m5l), a zero maximum standing in as
-  // 1 (its nu is 0 anyway); each nu then carries two roundings more than
-  // dxk m3 / m5 (relative 1e-16).  Products outside the normal range take
-  // the two divisions.
-  const double a = m5r > 0 ? m5r : 1.0, b = m5l > 0 ? m5l : 1.0;
-  const double p = a * b;
-  if (p > 1e-290 && p < 1e290) {
-    const double q = dxk / p;
-    nur = m5r > 0 ? (m3r * b) * q : 0.0;
-    nul = m5l > 0 ? (m3l * a) * q : 0.0;
+  // middle; max ignores NaN as the reference's "if (v > max)" does
+  const T t3 = tmax(D3[2], D3[3]);
+  const T t5 = tmax(tmax(D1[1], D1[2]), tmax(D1[3], D1[4]));
+  const T m3r = tmax(tmax(t3, D3[4]), T(0)), m3l = tmax(tmax(t3, D3[1]), T(0));
+  const T m5r = tmax(tmax(t5, D1[5]), T(0)), m5l = tmax(tmax(t5, D1[0]), T(0));
+  // one division for both: r = 1 / (m5r m5l), a zero maximum standing in
+  // as 1 (its nu is 0 anyway), then m3r / m5r = (m3r m5l) r, each ratio
+  // carrying two roundings more than a quotient.  The ratios are bounded
+  // (every third difference of a window is made of first differences of the
+  // same window: m3 <= 6 m5), so nothing overflows as long as r does;
+  // products outside the normal range take the two divisions.
+  const T lo = sizeof(T) == 8 ? T(1e-290) : T(1e-30), hi = sizeof(T) == 8 ? T(1e290) : T(1e30);
+  const T a = m5r > T(0) ? m5r : T(1), b = m5l > T(0) ? m5l : T(1);
+  const T p = a * b;
+  if (p > lo && p < hi) {
+    const T r = T(1) / p;
+    nur = m5r > T(0) ? dxk * ((m3r * b) * r) : T(0);
+    nul = m5l > T(0) ? dxk * ((m3l * a) * r) : T(0);
   } else {
-    nur = m5r > 0 ? dxk * m3r / m5r : 0.0;
-    nul = m5l > 0 ? dxk * m3l / m5l : 0.0;
+    nur = m5r > T(0) ? dxk * m3r / m5r : T(0);
+    nul = m5l > T(0) ? dxk * m3l / m5l : T(0);
   }
 }
 
 // nu from a window of q staged in LDS (centre index ci, stride s)
-template <typename SH>
-__device__ inline void nu_win(const SH* __restrict__ qb, int ci, int s, double dxk, double& nur, double& nul) {
-  double Q[7];
+template <typename SH, typename T>
+__device__ inline void nu_win(const SH* __restrict__ qb, int ci, int s, T dxk, T& nur, T& nul) {
+  T Q[7];
 #pragma unroll
-  for (int o = 0; o < 7; o++) Q[o] = qb[ci + (o - 3) * s];
-  nu7(Q, dxk, nur, nul);
+  for (int o = 0; o < 7; o++) Q[o] = (T)qb[ci + (o - 3) * s];
+  nu7<T>(Q, dxk, nur, nul);
 }
 
 // fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,
 // src/centdiff2.cu:158-356)
-// the state of one cell that its fluxes need
-template <int ND>
+// the state of one cell that its fluxes need, in the arithmetic type T
+template <int ND, typename T = double>
 struct CellState {
-  double m[ND], b[ND], bb[ND], rho, rhob, e, eb;
+  T m[ND], b[ND], bb[ND], rho, rhob, e, eb;
 };
 
-template <int ND, typename SH>
+template <int ND, typename SH, typename T = double>
 __device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
-                                 CellState<ND>& S) {
+                                 CellState<ND, T>& S) {
   const long long N = G.N;
 #pragma unroll
   for (int q = 0; q < ND; q++) {
-    S.m[q] = w[(long long)fMOM(q) * N + cc];
-    S.b[q] = w[(long long)fB(ND, q) * N + cc];
-    S.bb[q] = bg[(long long)bBB(q) * N + cc];
+    S.m[q] = (T)w[(long long)fMOM(q) * N + cc];
+    S.b[q] = (T)w[(long long)fB(ND, q) * N + cc];
+    S.bb[q] = (T)bg[(long long)bBB(q) * N + cc];
   }
-  S.rho = w[cc];
-  S.rhob = bg[(long long)bRHOB() * N + cc];
-  S.e = w[(long long)fE(ND) * N + cc];
-  S.eb = bg[(long long)bEB() * N + cc];
+  S.rho = (T)w[cc];
+  S.rhob = (T)bg[(long long)bRHOB() * N + cc];
+  S.e = (T)w[(long long)fE(ND) * N + cc];
+  S.eb = (T)bg[(long long)bEB() * N + cc];
 }
 
-template <int ND>
-__device__ inline void fluxes_of(const Geo& G, const CellState<ND>& S, int d, double* F) {
+template <int ND, typename T = double>
+__device__ inline void fluxes_of(const Geo& G, const CellState<ND, T>& S, int d, T* F) {
   // v = m/rho_t and p_t are recomputed from the cell's state (4 fewer
   // loads per flux cell than reading k_prim's copies); p_t follows
   // computept3, src/dervfields.cu:107-153
-  const double *m = S.m, *b = S.b, *bb = S.bb;
-  const double rho = S.rho, rhob = S.rhob, e = S.e, eb = S.eb;
-  const double rt = rho + rhob;
+  const T *m = S.m, *b = S.b, *bb = S.bb;
+  const T rho = S.rho, rhob = S.rhob, e = S.e, eb = S.eb;
+  const T rt = rho + rhob;
   // one reciprocal for the ND + 1 quotients by rho_t (each then carries
-  // one rounding more than a division: relative 1e-16)
-  const double rrt = 1.0 / rt;
-  double v[ND];
+  // one rounding more than a division)
+  const T rrt = T(1) / rt;
+  T v[ND];
 #pragma unroll
   for (int q = 0; q < ND; q++) v[q] = m[q] * rrt;
-  double mm = m[0] * m[0], b2 = b[0] * b[0], bbb = b[0] * bb[0];
+  T mm = m[0] * m[0], b2 = b[0] * b[0], bbb = b[0] * bb[0];
 #pragma unroll
   for (int q = 1; q < ND; q++) {
     mm = mm + m[q] * m[q];
     b2 = b2 + b[q] * b[q];
     bbb = bbb + b[q] * bb[q];
   }
-  const double pt = (G.gamma - 1.0) * (e - 0.5 * mm * rrt) - (G.gamma - 2.0) * (bbb + 0.5 * b2);
-  double vd = v[0], bd = b[0], bbd = bb[0];
+  const T gam = (T)G.gamma;
+  const T pt = (gam - T(1)) * (e - T(0.5) * mm * rrt) - (gam - T(2)) * (bbb + T(0.5) * b2);
+  T vd = v[0], bd = b[0], bbd = bb[0];
 #pragma unroll
   for (int q = 1; q < ND; q++)
     if (q == d) {
@@ -162,10 +175,10 @@ __device__ inline void fluxes_of(const Geo& G, const CellState<ND>& S, int d, do
   F[fRHO()] = vd * rho + rhob * vd;
 #pragma unroll
   for (int f = 0; f < ND; f++) {
-    const double mag = -(b[f] * bbd + bb[f] * bd) - b[f] * bd;
+    const T mag = -(b[f] * bbd + bb[f] * bd) - b[f] * bd;
     F[fMOM(f)] = vd * m[f] + (f == d ? pt + mag : mag);
   }
-  double bbv = bb[0] * v[0], bv = b[0] * v[0];
+  T bbv = bb[0] * v[0], bv = b[0] * v[0];
 #pragma unroll
   for (int q = 1; q < ND; q++) {
     bbv = bbv + bb[q] * v[q];
@@ -174,7 +187,7 @@ __device__ inline void fluxes_of(const Geo& G, const CellState<ND>& S, int d, do
   F[fE(ND)] = vd * e + ((vd * pt - bd * bbv - bbd * bv) + vd * eb - bd * bv);
 #pragma unroll
   for (int f = 0; f < ND; f++)
-    F[fB(ND, f)] = (f == d) ? 0.0 : vd * b[f] - (bd + bbd) * v[f] + bb[f] * vd;
+    F[fB(ND, f)] = (f == d) ? T(0) : vd * b[f] - (bd + bbd) * v[f] + bb[f] * vd;
 }
 
 template <int ND, typename SH>

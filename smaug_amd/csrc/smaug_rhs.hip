@@ -16,9 +16,12 @@
 //      background energy terms, which need gradients of v_f only.
 //   3. gravity, the stage update, the user source, the RK combination and
 //      the ghost-cell images (boundaries fused).
-// Neighbour values are read in the stencil precision SH: the FP64 state
-// itself, or (mixed precision) the FP32 shadows k_prim writes; all
-// arithmetic, the own-cell values and the update are FP64.
+// The stencils work in the stencil precision SH: FP64 (the state itself),
+// or, in mixed precision, FP32 -- the shadows k_prim writes, FP32 LDS boxes
+// and FP32 arithmetic, which halves the LDS and most of the registers of a
+// workgroup and so raises the number resident per CU.  The own-cell values,
+// the background terms, the stage update (base + h*acc), the driver and the
+// RK combination are FP64 in both.
 #include <math.h>
 
 #include "../../include/smaug_hip.h"
@@ -59,7 +62,7 @@ struct Lay {
                                     : imax((TX + 4) * TY, TX * (TY + 4));
   static constexpr int HYP = QN + 4 * RN;
   static constexpr int FLX = NP * FN;
-  // cell box: tile + 1-cell halo, FP64; box 0 rho_t, boxes 1..ND the
+  // cell box: tile + 1-cell halo; box 0 rho_t, boxes 1..ND the
   // components of the phase's vector (m in the momentum phases, b in the
   // field phases); it sits ahead of the area the phases share
   static constexpr int PX = TX + 2, PY = TY + 2, PZ = ND == 3 ? TZ + 2 : 1;
@@ -73,12 +76,12 @@ struct Lay {
 };
 
 // cell box of the tile plus one cell: rho_t = rho + rho_b (when with_rho)
-// and the ND components f0.. of the state, as the FP64 values the terms use
+// and the ND components f0.. of the state, in the stencil precision SH
 // (cells outside the grid hold 0 and are never read).  Loads and stores are
 // separate steps so the loads can share a round trip with a q box's.
-template <int ND>
+template <int ND, typename SH>
 struct CellBox {
-  double* CBX;  // null: no cell box with this q box
+  SH* CBX;  // null: no cell box with this q box
   bool with_rho;
   int f0;
 };
@@ -86,12 +89,12 @@ struct CellBox {
 template <int ND, typename SH>
 struct CellLoads {
   static constexpr int NK = (Lay<ND>::PN + 255) / 256;
-  double v[NK][ND + 1];
+  SH v[NK][ND + 1];
 };
 
 template <int ND, typename SH>
 __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
-                                  const CellBox<ND>& C, const int* o, CellLoads<ND, SH>& R) {
+                                  const CellBox<ND, SH>& C, const int* o, CellLoads<ND, SH>& R) {
   using L = Lay<ND>;
   constexpr int NK = CellLoads<ND, SH>::NK;
   int g[NK];  // cell of each slot, -1 outside the grid (or past the box)
@@ -104,14 +107,14 @@ __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const
   }
 #pragma unroll
   for (int k = 0; k < NK; k++) {
-    R.v[k][0] = C.with_rho && g[k] >= 0 ? (double)ws[g[k]] + bg[(long long)bRHOB() * G.N + g[k]] : 0.0;
+    R.v[k][0] = C.with_rho && g[k] >= 0 ? (SH)((double)ws[g[k]] + bg[(long long)bRHOB() * G.N + g[k]]) : SH(0);
 #pragma unroll
-    for (int d = 0; d < ND; d++) R.v[k][1 + d] = g[k] >= 0 ? (double)ws[(long long)(C.f0 + d) * G.N + g[k]] : 0.0;
+    for (int d = 0; d < ND; d++) R.v[k][1 + d] = g[k] >= 0 ? ws[(long long)(C.f0 + d) * G.N + g[k]] : SH(0);
   }
 }
 
 template <int ND, typename SH>
-__device__ inline void cells_store(const CellBox<ND>& C, const CellLoads<ND, SH>& R) {
+__device__ inline void cells_store(const CellBox<ND, SH>& C, const CellLoads<ND, SH>& R) {
   using L = Lay<ND>;
 #pragma unroll
   for (int k = 0; k < CellLoads<ND, SH>::NK; k++) {
@@ -159,7 +162,7 @@ __device__ inline void stage_q_edge(const Geo& G, const SH* __restrict__ qa, SH*
 template <int ND, typename SH>
 __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o,
                                const SH* __restrict__ ws = nullptr, const double* __restrict__ bg = nullptr,
-                               CellBox<ND> C = CellBox<ND>{nullptr, false, 0}, const SH* __restrict__ qa2 = nullptr,
+                               CellBox<ND, SH> C = CellBox<ND, SH>{nullptr, false, 0}, const SH* __restrict__ qa2 = nullptr,
                                SH* __restrict__ qb2 = nullptr) {
   using L = Lay<ND>;
   CellLoads<ND, SH> cl;
@@ -206,12 +209,16 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
   if (qa2) stage_q_edge<ND, SH>(G, qa2, qb2, o);
 }
 
+// FP32 stencils: at least five wavefronts per SIMD (96 VGPRs and a 12-byte
+// spill; measured faster at 256^3 than four without a spill or six with a
+// 72-byte spill).  FP64: three (151 VGPRs, 52 KB of LDS per workgroup).
 template <int ND, typename SH>
-__global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
+__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
+  using AT = SH;  // arithmetic type of the stencils (the own-cell update and the RK combination are FP64)
   constexpr int NP = L::NP;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
-  __shared__ double sm[L::SMEM];
+  __shared__ AT sm[L::SMEM];
   const int tid = threadIdx.x;
   if (A.zero_next && blockIdx.x == 0 && tid < 3) A.zero_next[tid] = 0ull;
   // XCD-aware tile order: the hardware deals consecutive workgroups to the
@@ -236,33 +243,33 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   const double* __restrict__ bg = A.bg;
   const SH* __restrict__ pr = (const SH*)A.prim;
   const unsigned T = G.terms;
-  double acc[NP];
+  AT acc[NP];
 #pragma unroll
-  for (int f = 0; f < NP; f++) acc[f] = 0.0;
-  double dx[ND], rdx[ND];
+  for (int f = 0; f < NP; f++) acc[f] = AT(0);
+  AT dx[ND], rdx[ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
     if (A.dx1) {  // tensor-product grid: one spacing per coordinate
       const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
-      dx[d] = valid ? A.dx1[od + x[d]] : 1.0;
-      rdx[d] = valid ? A.rdx1[od + x[d]] : 1.0;
+      dx[d] = valid ? (AT)A.dx1[od + x[d]] : AT(1);
+      rdx[d] = valid ? (AT)A.rdx1[od + x[d]] : AT(1);
     } else {
-      dx[d] = valid ? A.delx[(long long)d * N + c] : 1.0;
-      rdx[d] = valid ? A.rdelx[(long long)d * N + c] : 1.0;
+      dx[d] = valid ? (AT)A.delx[(long long)d * N + c] : AT(1);
+      rdx[d] = valid ? (AT)A.rdelx[(long long)d * N + c] : AT(1);
     }
   }
 
-  double* const SDX = sm + L::CB + L::SHR;  // spacing table (tensor-product grids)
+  AT* const SDX = sm + L::CB + L::SHR;  // spacing table (tensor-product grids)
   if (A.dx1 && tid < L::DXT) {  // visible after the first barrier
     const int d = tid < TX ? 0 : (tid < TX + TY ? 1 : 2);
     const int xx = o[d] + tid - (d == 0 ? 0 : (d == 1 ? TX : TX + TY));
     const int od = d == 0 ? 0 : (d == 1 ? G.n[0] : G.n[0] + G.n[1]);
     const bool ing = d < ND && xx < G.n[d];
-    SDX[tid] = ing ? A.dx1[od + xx] : 1.0;
-    SDX[L::DXT + tid] = ing ? A.rdx1[od + xx] : 1.0;
+    SDX[tid] = ing ? (AT)A.dx1[od + xx] : AT(1);
+    SDX[L::DXT + tid] = ing ? (AT)A.rdx1[od + xx] : AT(1);
   }
 
-  double* const CBX = sm;  // cell box (tile + 1)
+  AT* const CBX = sm;  // cell box (tile + 1)
   const int pc = (lc[0] + 1) + L::PX * ((lc[1] + 1) + L::PY * (ND == 3 ? lc[2] + 1 : 0));
   const int ps[3] = {1, L::PX, L::PX * L::PY};
   const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
@@ -270,7 +277,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
 
   // ------------------------------------------------------------ fluxes
   if (T & SMAUG_T_FLUX) {
-    double* FB = sm + L::CB;
+    AT* FB = sm + L::CB;
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       const int ex[3] = {d == 0 ? 2 : 0, d == 1 ? 2 : 0, d == 2 ? 2 : 0};
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
       // out together, then the fluxes are formed and stored
       static_assert(L::FN <= 512, "two flux cells per thread");
       const int R = rx * ry * rz;
-      CellState<ND> S[2];
+      CellState<ND, AT> S[2];
       bool in[2];
 #pragma unroll
       for (int k = 0; k < 2; k++) {
@@ -287,18 +294,18 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
         const int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
         in[k] = l < R && in_grid<ND>(G, xr);
-        if (in[k]) load_cell<ND, SH>(G, ws, bg, gix(G, xr[0], xr[1], xr[2]), S[k]);
+        if (in[k]) load_cell<ND, SH, AT>(G, ws, bg, gix(G, xr[0], xr[1], xr[2]), S[k]);
       }
 #pragma unroll
       for (int k = 0; k < 2; k++) {
         const int l = tid + 256 * k;
         if (l < R) {
-          double F[NP];
+          AT F[NP];
           if (in[k]) {
-            fluxes_of<ND>(G, S[k], d, F);
+            fluxes_of<ND, AT>(G, S[k], d, F);
           } else {
 #pragma unroll
-            for (int f = 0; f < NP; f++) F[f] = 0.0;
+            for (int f = 0; f < NP; f++) F[f] = AT(0);
           }
 #pragma unroll
           for (int f = 0; f < NP; f++) FB[f * L::FN + l] = F[f];
@@ -308,10 +315,10 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
       if (valid && !z4(G, zFLUX(ND), d, x[d])) {
         const int s = d == 0 ? 1 : (d == 1 ? rx : rx * ry);
         const int li = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
-        const double inv = rdx[d] * (1.0 / 12.0);
+        const AT inv = rdx[d] * AT(1.0 / 12.0);
 #pragma unroll
         for (int f = 0; f < NP; f++) {
-          const double* Fp = FB + f * L::FN + li;
+          const AT* Fp = FB + f * L::FN + li;
           acc[f] -= ((8 * Fp[s] - 8 * Fp[-s]) + (Fp[-2 * s] - Fp[2 * s])) * inv;
         }
       }
@@ -321,66 +328,67 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
 
   const double rho_c = valid ? w[c] : 0.0;
   const double rhob_c = valid ? bg[(long long)bRHOB() * N + c] : 1.0;
-  const double rt_c = rho_c + rhob_c;
+  const AT rt_c = (AT)(rho_c + rhob_c);
 
   // ------------------------------------------------------------ hyperdiffusion + energy terms
-  SH* qb = reinterpret_cast<SH*>(sm + L::CB);                                        // q box
-  double* NR = sm + L::CB + (L::QN * sizeof(SH) + sizeof(double) - 1) / sizeof(double);  // region arrays
-  double* RG[4] = {NR, NR + L::RN, NR + 2 * L::RN, NR + 3 * L::RN};  // GA/GB of the two cross directions
+  SH* qb = sm + L::CB;                                 // q box
+  AT* NR = sm + L::CB + L::QN;                         // region arrays
+  AT* RG[4] = {NR, NR + L::RN, NR + 2 * L::RN, NR + 3 * L::RN};  // GA/GB of the two cross directions
   // tile-cell index in the q box and its strides
   const int qc = (lc[0] + 3) + L::QX * ((lc[1] + 3) + L::QY * (ND == 3 ? lc[2] + 3 : 0));
   const int qs[3] = {1, L::QX, L::QX * L::QY};
 
-  double cm[ND];
+  AT cm[ND];
 #pragma unroll
-  for (int d = 0; d < ND; d++) cm[d] = __longlong_as_double((long long)A.cmax_bits[d]);
+  for (int d = 0; d < ND; d++) cm[d] = (AT)__longlong_as_double((long long)A.cmax_bits[d]);
 
   // rho (src/hyperdifrhosource1.cu) and e_int (src/hyperdifesource1.cu):
   // both boxes are staged in one round trip, the e_int box after the rho box
   const bool h_rho = G.hyper && (T & SMAUG_T_HRHO), h_e = G.hyper && (T & SMAUG_T_HE);
-  static_assert(2 * L::QN * sizeof(SH) <= L::SHR * sizeof(double), "two q boxes in the shared area");
+  static_assert(2 * L::QN <= L::SHR, "two q boxes in the shared area");
   if (h_rho || h_e) {
     const SH* qe = pr + (long long)pQE(ND) * N;
     SH* qbe = h_rho ? qb + L::QN : qb;
-    stage_q<ND, SH>(G, h_rho ? ws : qe, qb, o, ws, bg, CellBox<ND>{nullptr, false, 0}, h_rho && h_e ? qe : nullptr,
+    stage_q<ND, SH>(G, h_rho ? ws : qe, qb, o, ws, bg, CellBox<ND, SH>{nullptr, false, 0}, h_rho && h_e ? qe : nullptr,
                     h_rho && h_e ? qbe : nullptr);
     __syncthreads();
     if (valid && h_rho) {
 #pragma unroll
       for (int dim = 0; dim < ND; dim++) {
-        double nur, nul;
-        nu_win(qb, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fRHO()], nur, nul);
+        AT nur, nul;
+        nu_win<SH, AT>(qb, qc, qs[dim], dx[dim] * cm[dim] * (AT)G.chyp[fRHO()], nur, nul);
         const bool z = z1(G, zRHO, dim, x[dim]);
-        const double q0 = (double)qb[qc];
-        const double gr = z ? 0.0 : ((double)qb[qc + qs[dim]] - q0) * rdx[dim];
-        const double gl = z ? 0.0 : (q0 - (double)qb[qc - qs[dim]]) * rdx[dim];
+        const AT q0 = qb[qc];
+        const AT gr = z ? AT(0) : (qb[qc + qs[dim]] - q0) * rdx[dim];
+        const AT gl = z ? AT(0) : (q0 - qb[qc - qs[dim]]) * rdx[dim];
         acc[fRHO()] += (nur * gr - nul * gl) * rdx[dim];
       }
     }
     if (valid && h_e) {
 #pragma unroll
       for (int dim = 0; dim < ND; dim++) {
-        double nur, nul;
-        nu_win(qbe, qc, qs[dim], dx[dim] * cm[dim] * G.chyp[fE(ND)], nur, nul);
+        AT nur, nul;
+        nu_win<SH, AT>(qbe, qc, qs[dim], dx[dim] * cm[dim] * (AT)G.chyp[fE(ND)], nur, nul);
         const bool z = z1(G, zTMP1, dim, x[dim]);
-        const double q0 = (double)qbe[qc];
-        const double gr = z ? 0.0 : ((double)qbe[qc + qs[dim]] - q0) * rdx[dim];
-        const double gl = z ? 0.0 : (q0 - (double)qbe[qc - qs[dim]]) * rdx[dim];
+        const AT q0 = qbe[qc];
+        const AT gr = z ? AT(0) : (qbe[qc + qs[dim]] - q0) * rdx[dim];
+        const AT gl = z ? AT(0) : (q0 - qbe[qc - qs[dim]]) * rdx[dim];
         acc[fE(ND)] += (nur * gr - nul * gl) * rdx[dim];
       }
     }
     __syncthreads();
   }
 
-  double ptb = 0.0, bbc[ND];
+  AT ptb = AT(0), bbc[ND];  // background terms, formed in FP64
   if (want_eterm && valid) {
-    double bb2 = 0.0;
+    double bb2 = 0.0, bbd[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      bbc[d] = bg[(long long)bBB(d) * N + c];
-      bb2 = d == 0 ? bbc[d] * bbc[d] : bb2 + bbc[d] * bbc[d];
+      bbd[d] = bg[(long long)bBB(d) * N + c];
+      bbc[d] = (AT)bbd[d];
+      bb2 = d == 0 ? bbd[d] * bbd[d] : bb2 + bbd[d] * bbd[d];
     }
-    ptb = (G.gamma - 1) * bg[(long long)bEB() * N + c] - 0.5 * (G.gamma - 2) * bb2;
+    ptb = (AT)((G.gamma - 1) * bg[(long long)bEB() * N + c] - 0.5 * (G.gamma - 2) * bb2);
   }
 
   // momentum phases (q = v_f): energy terms, diagonal and off-diagonal sources
@@ -389,18 +397,18 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
     for (int f = 0; f < ND; f++) {
       // the first box brings rho_t and m on the cell box with it
       stage_q<ND, SH>(G, pr + (long long)pV(f) * N, qb, o, ws, bg,
-                      CellBox<ND>{f == 0 && want_mom ? CBX : nullptr, true, fMOM(0)});
+                      CellBox<ND, SH>{f == 0 && want_mom ? CBX : nullptr, true, fMOM(0)});
       __syncthreads();
       if (want_eterm && valid) {  // addenergyterms_cd2 for dir f
-        double gf[ND];
+        AT gf[ND];
 #pragma unroll
         for (int i2 = 0; i2 < ND; i2++) {
           const int s = qs[i2];
           gf[i2] = z4(G, zVEL(f), i2, x[i2])
-                       ? 0.0
-                       : ((8 * (double)qb[qc + s] - 8 * (double)qb[qc - s]) + ((double)qb[qc - 2 * s] - (double)qb[qc + 2 * s])) * (rdx[i2] * (1.0 / 12.0));
+                       ? AT(0)
+                       : ((8 * qb[qc + s] - 8 * qb[qc - s]) + (qb[qc - 2 * s] - qb[qc + 2 * s])) * (rdx[i2] * AT(1.0 / 12.0));
         }
-        double t = -ptb * gf[f];
+        AT t = -ptb * gf[f];
 #pragma unroll
         for (int i2 = 0; i2 < ND; i2++) t += bbc[i2] * bbc[f] * gf[i2];
         acc[fE(ND)] += t;
@@ -414,19 +422,19 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1];
         const int sf = f == 0 ? 1 : (f == 1 ? rx : rx * ry);
         const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
-        const double Kc = G.chyp[fMOM(f)];
-        double nur[ND], nul[ND];
+        const AT Kc = (AT)G.chyp[fMOM(f)];
+        AT nur[ND], nul[ND];
 #pragma unroll
         for (int d = 0; d < ND; d++) {
-          nur[d] = nul[d] = 0.0;
-          if (valid) nu_win(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
+          nur[d] = nul[d] = AT(0);
+          if (valid) nu_win<SH, AT>(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
           if (d != f) {
             const int k = d < f ? d : d - 1;
-            double g7 = 0, g8 = 0;
+            AT g7 = 0, g8 = 0;
             if (valid) {
-              const double t5 =
-                  z1(G, zTMP4, d, x[d]) ? 0.0 : ((double)qb[qc + qs[d]] - (double)qb[qc - qs[d]]) * rdx[d] * 0.5;
-              const double t6 = t5 * (nur[d] + nul[d]) * 0.25;
+              const AT t5 =
+                  z1(G, zTMP4, d, x[d]) ? AT(0) : (qb[qc + qs[d]] - qb[qc - qs[d]]) * rdx[d] * AT(0.5);
+              const AT t6 = t5 * (nur[d] + nul[d]) * AT(0.25);
               g7 = rt_c * t6;
               g8 = t6 * CBX[(1 + d) * L::PN + pc];
             }
@@ -450,23 +458,23 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
             const bool in = interior_c<ND>(G, xr);
             const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
             const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
-            const double rt = in ? CBX[pi] : 0.0;
+            const AT rt = in ? CBX[pi] : AT(0);
             const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
 #pragma unroll
             for (int d = 0; d < ND; d++) {
               if (d == f) continue;
               const int k = d < f ? d : d - 1;
-              double g7 = 0, g8 = 0;
+              AT g7 = 0, g8 = 0;
               if (in) {
                 // d != f: the border cell's coordinate along d is a tile coordinate
                 const int ti = (d == 0 ? 0 : (d == 1 ? TX : TX + TY)) + rel[d];
-                const double dxr = A.dx1 ? SDX[ti] : A.delx[(long long)d * N + gq];
-                const double rdxr = A.dx1 ? SDX[L::DXT + ti] : A.rdelx[(long long)d * N + gq];
-                double nr, nl;
-                nu_win(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
-                const double t5 =
-                    z1(G, zTMP4, d, xr[d]) ? 0.0 : ((double)qb[qi + qs[d]] - (double)qb[qi - qs[d]]) * rdxr * 0.5;
-                const double t6 = t5 * (nr + nl) * 0.25;
+                const AT dxr = A.dx1 ? SDX[ti] : (AT)A.delx[(long long)d * N + gq];
+                const AT rdxr = A.dx1 ? SDX[L::DXT + ti] : (AT)A.rdelx[(long long)d * N + gq];
+                AT nr, nl;
+                nu_win<SH, AT>(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
+                const AT t5 =
+                    z1(G, zTMP4, d, xr[d]) ? AT(0) : (qb[qi + qs[d]] - qb[qi - qs[d]]) * rdxr * AT(0.5);
+                const AT t6 = t5 * (nr + nl) * AT(0.25);
                 g7 = rt * t6;
                 g8 = t6 * CBX[(1 + d) * L::PN + pi];
               }
@@ -477,21 +485,21 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         }
         __syncthreads();
         if (valid) {
-          const double* mf = CBX + (1 + f) * L::PN + pc;
-          const double mc = mf[0];
+          const AT* mf = CBX + (1 + f) * L::PN + pc;
+          const AT mc = mf[0];
 #pragma unroll
           for (int d = 0; d < ND; d++) {
             const int s = qs[d];
             const int sp = ps[d];
             const bool z = z1(G, zTMP4, d, x[d]);
-            const double q0 = (double)qb[qc];
-            const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[d];
-            const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[d];
-            const double rtp = CBX[pc + sp], rtm = CBX[pc - sp];
-            const double rr = (rt_c + rtp) * 0.5, rl = (rt_c + rtm) * 0.5;
-            const double mr = (mc + mf[sp]) * 0.5, ml = (mc + mf[-sp]) * 0.5;
-            const double dmv = (rr * nur[d] * gr - rl * nul[d] * gl) * (rdx[d] * 0.5);
-            const double dev = (mr * nur[d] * gr - ml * nul[d] * gl) * (rdx[d] * 0.5);
+            const AT q0 = qb[qc];
+            const AT gr = z ? AT(0) : (qb[qc + s] - q0) * rdx[d];
+            const AT gl = z ? AT(0) : (q0 - qb[qc - s]) * rdx[d];
+            const AT rtp = CBX[pc + sp], rtm = CBX[pc - sp];
+            const AT rr = (rt_c + rtp) * AT(0.5), rl = (rt_c + rtm) * AT(0.5);
+            const AT mr = (mc + mf[sp]) * AT(0.5), ml = (mc + mf[-sp]) * AT(0.5);
+            const AT dmv = (rr * nur[d] * gr - rl * nul[d] * gl) * (rdx[d] * AT(0.5));
+            const AT dev = (mr * nur[d] * gr - ml * nul[d] * gl) * (rdx[d] * AT(0.5));
             acc[fMOM(f)] += dmv;
             acc[fE(ND)] += dev;
             if (d == f) {
@@ -499,10 +507,10 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
               acc[fE(ND)] += dev;
             } else {
               const int k = d < f ? d : d - 1;
-              const double* GAk = RG[2 * k];
-              const double* GBk = RG[2 * k + 1];
-              const double o7 = z1(G, zTMP7, f, x[f]) ? 0.0 : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * 0.5);
-              const double o8 = z1(G, zTMP8, f, x[f]) ? 0.0 : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * 0.5);
+              const AT* GAk = RG[2 * k];
+              const AT* GBk = RG[2 * k + 1];
+              const AT o7 = z1(G, zTMP7, f, x[f]) ? AT(0) : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * AT(0.5));
+              const AT o8 = z1(G, zTMP8, f, x[f]) ? AT(0) : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * AT(0.5));
               acc[fMOM(d)] += o7;
               acc[fE(ND)] += o8;
             }
@@ -522,25 +530,25 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
       // the first box brings b on the cell box with it (the momentum
       // phases are done with the cell box: each ended with a barrier)
       stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o, ws, bg,
-                      CellBox<ND>{f == 0 ? CBX : nullptr, false, fB(ND, 0)});
+                      CellBox<ND, SH>{f == 0 ? CBX : nullptr, false, fB(ND, 0)});
       __syncthreads();
       const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
       const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1];
       const int sf = f == 0 ? 1 : (f == 1 ? rx : rx * ry);
       const int ri = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
-      const double Kc = G.chyp[fB(ND, f)];
-      double nur[ND], nul[ND];
+      const AT Kc = (AT)G.chyp[fB(ND, f)];
+      AT nur[ND], nul[ND];
 #pragma unroll
       for (int d = 0; d < ND; d++) {
-        nur[d] = nul[d] = 0.0;
+        nur[d] = nul[d] = AT(0);
         if (d == f) continue;
         const int k = d < f ? d : d - 1;
-        double g3 = 0, g5 = 0;
+        AT g3 = 0, g5 = 0;
         if (valid) {
-          nu_win(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
-          const double t2 =
-              z1(G, zTMP1, d, x[d]) ? 0.0 : ((double)qb[qc + qs[d]] - (double)qb[qc - qs[d]]) * rdx[d] * 0.5;
-          const double t3 = t2 * (nul[d] + nur[d]) * 0.5;
+          nu_win<SH, AT>(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
+          const AT t2 =
+              z1(G, zTMP1, d, x[d]) ? AT(0) : (qb[qc + qs[d]] - qb[qc - qs[d]]) * rdx[d] * AT(0.5);
+          const AT t3 = t2 * (nul[d] + nur[d]) * AT(0.5);
           g3 = t3;
           g5 = t3 * CBX[(1 + d) * L::PN + pc];
         }
@@ -568,17 +576,17 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
           for (int d = 0; d < ND; d++) {
             if (d == f) continue;
             const int k = d < f ? d : d - 1;
-            double g3 = 0, g5 = 0;
+            AT g3 = 0, g5 = 0;
             if (in) {
               // d != f: the border cell's coordinate along d is a tile coordinate
               const int ti = (d == 0 ? 0 : (d == 1 ? TX : TX + TY)) + rel[d];
-              const double dxr = A.dx1 ? SDX[ti] : A.delx[(long long)d * N + gq];
-              const double rdxr = A.dx1 ? SDX[L::DXT + ti] : A.rdelx[(long long)d * N + gq];
-              double nr, nl;
-              nu_win(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
-              const double t2 =
-                  z1(G, zTMP1, d, xr[d]) ? 0.0 : ((double)qb[qi + qs[d]] - (double)qb[qi - qs[d]]) * rdxr * 0.5;
-              const double t3 = t2 * (nl + nr) * 0.5;
+              const AT dxr = A.dx1 ? SDX[ti] : (AT)A.delx[(long long)d * N + gq];
+              const AT rdxr = A.dx1 ? SDX[L::DXT + ti] : (AT)A.rdelx[(long long)d * N + gq];
+              AT nr, nl;
+              nu_win<SH, AT>(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
+              const AT t2 =
+                  z1(G, zTMP1, d, xr[d]) ? AT(0) : (qb[qi + qs[d]] - qb[qi - qs[d]]) * rdxr * AT(0.5);
+              const AT t3 = t2 * (nl + nr) * AT(0.5);
               g3 = t3;
               g5 = t3 * CBX[(1 + d) * L::PN + pi];
             }
@@ -593,18 +601,18 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
         for (int d = 0; d < ND; d++) {
           if (d == f) continue;
           const int k = d < f ? d : d - 1;
-          const double* GAk = RG[2 * k];
-          const double* GBk = RG[2 * k + 1];
-          const double o3 = z1(G, zTMP3, f, x[f]) ? 0.0 : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * 0.5);
-          const double o5 = z1(G, zTMP5, f, x[f]) ? 0.0 : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * 0.5);
-          acc[fB(ND, d)] += -1.0 * o3;
-          acc[fE(ND)] += -1.0 * o5;
+          const AT* GAk = RG[2 * k];
+          const AT* GBk = RG[2 * k + 1];
+          const AT o3 = z1(G, zTMP3, f, x[f]) ? AT(0) : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * AT(0.5));
+          const AT o5 = z1(G, zTMP5, f, x[f]) ? AT(0) : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * AT(0.5));
+          acc[fB(ND, d)] += -o3;
+          acc[fE(ND)] += -o5;
           const int s = qs[d];
           const bool z = z1(G, zTMP1, d, x[d]);
-          const double q0 = (double)qb[qc];
-          const double gr = z ? 0.0 : ((double)qb[qc + s] - q0) * rdx[d];
-          const double gl = z ? 0.0 : (q0 - (double)qb[qc - s]) * rdx[d];
-          const double br = (q0 + (double)qb[qc + s]) * 0.5, bl = (q0 + (double)qb[qc - s]) * 0.5;
+          const AT q0 = qb[qc];
+          const AT gr = z ? AT(0) : (qb[qc + s] - q0) * rdx[d];
+          const AT gl = z ? AT(0) : (q0 - qb[qc - s]) * rdx[d];
+          const AT br = (q0 + qb[qc + s]) * AT(0.5), bl = (q0 + qb[qc - s]) * AT(0.5);
           acc[fB(ND, f)] += (gr * nur[d] - gl * nul[d]) * rdx[d];
           acc[fE(ND)] += (br * gr * nur[d] - bl * gl * nul[d]) * rdx[d];
         }
@@ -618,15 +626,15 @@ __global__ __launch_bounds__(256) void k_rhs_tiled(Geo G, RhsArgs A) {
   if (T & SMAUG_T_GRAV) {
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      acc[fMOM(d)] += G.g[d] * rho_c;
-      acc[fE(ND)] += rho_c * (G.g[d] * w[(long long)fMOM(d) * N + c]) / rt_c;
+      acc[fMOM(d)] += (AT)(G.g[d] * rho_c);
+      acc[fE(ND)] += (AT)(rho_c * (G.g[d] * w[(long long)fMOM(d) * N + c]) / (rho_c + rhob_c));
     }
   }
   // ------------------------------------------------------------ update, source, RK, ghosts
   const double h = A.hfrac * A.dts[0];
   double out[NP];
 #pragma unroll
-  for (int f = 0; f < NP; f++) out[f] = A.base[(long long)f * N + c] + h * acc[f];
+  for (int f = 0; f < NP; f++) out[f] = A.base[(long long)f * N + c] + h * (double)acc[f];
   if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_VERTICAL) {
     const double* dv = G.drv;
     const double zp = A.pos[c] - dv[2];

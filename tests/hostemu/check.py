@@ -5,10 +5,12 @@ host build, so the product binding (smaug_amd.engine) drives the same C ABI
 and kernels the GPU build compiles.
 
   python -m tests.hostemu.check fwd  <case>            forward vs oracle
+  python -m tests.hostemu.check mixed <case>           mixed precision vs oracle
   python -m tests.hostemu.check adj  <case> <terms>    adjoint vs oracle FD
   python -m tests.hostemu.check autograd <rkon>        smaug_amd.adjoint path
 
-case: s2r (2D stratified RK4), s2e (2D Euler), s3r (3D RK4), ot (2D periodic).
+case: s2r (2D stratified RK4), s2e (2D Euler), s3r (3D RK4), s3e (3D Euler,
+mixed only), ot (2D periodic).
 """
 from __future__ import annotations
 
@@ -130,6 +132,30 @@ def check_fwd(case: str):
     return {"rel_l2": compare(rp, got, ref).tolist()}
 
 
+def check_mixed(case: str):
+    """Mixed precision (FP32 shadows and stencil arithmetic) against the
+    FP64 oracle, with the step count of the GPU mixed-precision tests."""
+    if case == "ot":
+        rp, w, pos = ot_case(ni=32, rkon=1)
+        nsteps = 30
+    elif case == "s2r":
+        rp, w, pos = strat_case(ni=(24, 16), ndim=2, rkon=1)
+        nsteps = 20
+    else:
+        rp, w, pos = strat_case(ni=(14, 10, 12), rkon=0 if case == "s3e" else 1)
+        nsteps = 10
+    rp.precision = P.PREC_MIXED
+    e = Engine(rp, w, pos)
+    e.step(nsteps)
+    got = e.state()
+    e.close()
+    o = oracle.Oracle(rp, w, pos)
+    o.step(nsteps)
+    ref = o.state()
+    o.close()
+    return {"rel_l2": compare(rp, got, ref).tolist(), "finite": bool(np.all(np.isfinite(got)))}
+
+
 def check_adj(case: str, terms: int):
     rp, w, pos, amap, il, nsteps = make_case(case, terms)
     gw, ga = engine_adjoint(rp, w, pos, amap, nsteps, il)
@@ -213,6 +239,8 @@ def main(argv):
     what = argv[0]
     if what == "fwd":
         r = check_fwd(argv[1])
+    elif what == "mixed":
+        r = check_mixed(argv[1])
     elif what == "adj":
         r = check_adj(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
     else:

@@ -4,7 +4,8 @@ thread as a fiber and guard pages around every buffer, so
 
   * an out-of-bounds access of any kernel ends the check (run twice: buffers
     placed against their trailing and their leading guard page);
-  * the forward step matches the oracle (relative L2 <= 1e-12 per variable);
+  * the forward step matches the oracle (relative L2 <= 1e-12 per variable),
+    and the mixed-precision step stays within 1e-5 of it;
   * the adjoint step (smaug_adjoint_step) matches Richardson-extrapolated
     central differences (h = 1e-6 relative) of the oracle with its
     hyperdiffusion coefficients replayed, to 1e-6 relative, per evolved
@@ -57,6 +58,15 @@ def close(pair):
 def test_forward_matches_oracle(hostlib, case, align):
     r = run(hostlib, "fwd", case, align=align)
     assert max(r["rel_l2"]) <= 1e-12, r
+
+
+@pytest.mark.parametrize("case", ["s3e", "s3r", "s2r", "ot"])
+def test_mixed_precision_matches_oracle(hostlib, case):
+    """The FP32 stencil path (shadows, LDS boxes and arithmetic) stays finite
+    and within 1e-5 relative L2 per variable of the FP64 oracle."""
+    r = run(hostlib, "mixed", case)
+    assert r["finite"], r
+    assert max(r["rel_l2"]) <= 1e-5, r
 
 
 @pytest.mark.parametrize("case", ["s2r", "s2e", "s3r", "ot"])
