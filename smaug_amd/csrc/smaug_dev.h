@@ -129,6 +129,40 @@ __host__ __device__ inline int bc_images(int type, int y, int n, int nd, int* ou
   return m;
 }
 
+// The same images as (count, first, second) for n >= 8, where a coordinate
+// has at most two: closed forms, no array written at a run-time index.
+struct Img2 {
+  int m, g0, g1;
+};
+__host__ __device__ inline Img2 bc_images2(int type, int y, int n, int nd) {
+  if (type == BC_FIXED) return Img2{0, 0, 0};
+  if (type == BC_PERIODIC) {
+    if (y == n - 4) return Img2{1, 0, 0};
+    if (y == n - 3) return Img2{1, 1, 0};
+    if (y == 2) return Img2{1, n - 2, 0};
+    if (y == 3) return Img2{1, n - 1, 0};
+    return Img2{0, 0, 0};
+  }
+  if (type == BC_CONT) {
+    if (y == 2) return Img2{2, 0, 1};
+    if (y == n - 3) return Img2{2, n - 2, n - 1};
+    return Img2{0, 0, 0};
+  }
+  if (nd == 2 && (type == BC_SYMM || type == BC_ASYMM)) {
+    if (y == 3) return Img2{1, 0, 0};
+    if (y == 2) return Img2{1, 1, 0};
+    if (y == n - 4) return Img2{1, n - 1, 0};
+    if (y == n - 3) return Img2{1, n - 2, 0};
+    return Img2{0, 0, 0};
+  }
+  // contcd4; symm/asymm in 3D
+  if (y == 4) return Img2{1, 0, 0};
+  if (y == 3) return Img2{1, 1, 0};
+  if (y == n - 5) return Img2{1, n - 1, 0};
+  if (y == n - 4) return Img2{1, n - 2, 0};
+  return Img2{0, 0, 0};
+}
+
 // zeroing of grad3dn at interior cells: first/last two interior layers
 __device__ inline bool z4(const Geo& G, int idx, int d, int x) {
   const int n = G.n[d];

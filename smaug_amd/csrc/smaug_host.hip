@@ -146,6 +146,11 @@ static int check_params(const smaug_params* p) {
   for (int d = 0; d < p->ndim; d++)
     if (p->n[d] < 8) return fail(SMAUG_E_ARG, "n[%d] must be >= 8 (got %d)", d, p->n[d]);
   if (p->ndim == 2 && p->n[2] != 1) return fail(SMAUG_E_ARG, "2D runs need n[2] == 1");
+  {  // the stencil kernels address a field with 32-bit byte offsets
+    const long long cells = (long long)p->n[0] * p->n[1] * (p->ndim == 3 ? p->n[2] : 1);
+    if (cells >= (1ll << 29))
+      return fail(SMAUG_E_UNSUPPORTED, "%lld cells: grids of 2^29 cells or more are not supported", cells);
+  }
   if (p->divbon) return fail(SMAUG_E_UNSUPPORTED, "divbon=1 is not supported");
   if (p->rkon != 0 && p->rkon != 1) return fail(SMAUG_E_ARG, "rkon must be 0 or 1");
   if (p->precision != SMAUG_PREC_FP64 && p->precision != SMAUG_PREC_MIXED)

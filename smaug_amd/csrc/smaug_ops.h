@@ -12,6 +12,26 @@ __device__ inline long long gix(const Geo& G, int a, int b, int c) {
   return ((long long)c * G.n[1] + b) * G.n[0] + a;
 }
 
+// The same cell index as a 32-bit value: a context holds fewer than 2^29
+// cells (checked at creation), so every byte offset within one field fits
+// 32 bits.  at(base, i) reads element i of a field whose base pointer is
+// the same for the whole wavefront, as base + 32-bit byte offset: the form
+// of the global load with a scalar base and a 32-bit vector offset (no
+// 64-bit address arithmetic per lane); put(base, i, v) writes it.
+__device__ inline unsigned cix(const Geo& G, int a, int b, int c) {
+  return ((unsigned)c * (unsigned)G.n[1] + (unsigned)b) * (unsigned)G.n[0] + (unsigned)a;
+}
+
+template <typename T>
+__device__ inline T at(const T* __restrict__ base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+
+template <typename T>
+__device__ inline void put(T* __restrict__ base, unsigned i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T)) = v;
+}
+
 // |x| and max(a, b) in the arithmetic type of the call (FP64, or FP32 for
 // the mixed-precision stencils); max ignores a NaN operand
 __device__ inline double tabs(double x) { return fabs(x); }
@@ -126,19 +146,19 @@ struct CellState {
 };
 
 template <int ND, typename SH, typename T = double>
-__device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
+__device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, unsigned cc,
                                  CellState<ND, T>& S) {
   const long long N = G.N;
 #pragma unroll
   for (int q = 0; q < ND; q++) {
-    S.m[q] = (T)w[(long long)fMOM(q) * N + cc];
-    S.b[q] = (T)w[(long long)fB(ND, q) * N + cc];
-    S.bb[q] = (T)bg[(long long)bBB(q) * N + cc];
+    S.m[q] = (T)at(w + (long long)fMOM(q) * N, cc);
+    S.b[q] = (T)at(w + (long long)fB(ND, q) * N, cc);
+    S.bb[q] = (T)at(bg + (long long)bBB(q) * N, cc);
   }
-  S.rho = (T)w[cc];
-  S.rhob = (T)bg[(long long)bRHOB() * N + cc];
-  S.e = (T)w[(long long)fE(ND) * N + cc];
-  S.eb = (T)bg[(long long)bEB() * N + cc];
+  S.rho = (T)at(w, cc);
+  S.rhob = (T)at(bg + (long long)bRHOB() * N, cc);
+  S.e = (T)at(w + (long long)fE(ND) * N, cc);
+  S.eb = (T)at(bg + (long long)bEB() * N, cc);
 }
 
 template <int ND, typename T = double>
@@ -194,7 +214,7 @@ template <int ND, typename SH>
 __device__ inline void fluxes2(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, long long cc,
                                int d, double* F) {
   CellState<ND> S;
-  load_cell<ND, SH>(G, w, bg, cc, S);
+  load_cell<ND, SH>(G, w, bg, (unsigned)cc, S);
   fluxes_of<ND>(G, S, d, F);
 }
 

@@ -103,13 +103,15 @@ __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const
     const int l = threadIdx.x + 256 * k;
     const int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
     const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
-    g[k] = l < L::PN && in_grid<ND>(G, x) ? (int)gix(G, x[0], x[1], x[2]) : -1;
+    g[k] = l < L::PN && in_grid<ND>(G, x) ? (int)cix(G, x[0], x[1], x[2]) : -1;
   }
 #pragma unroll
   for (int k = 0; k < NK; k++) {
-    R.v[k][0] = C.with_rho && g[k] >= 0 ? (SH)((double)ws[g[k]] + bg[(long long)bRHOB() * G.N + g[k]]) : SH(0);
+    R.v[k][0] = C.with_rho && g[k] >= 0 ? (SH)((double)at(ws, (unsigned)g[k]) + at(bg + (long long)bRHOB() * G.N, (unsigned)g[k]))
+                                        : SH(0);
 #pragma unroll
-    for (int d = 0; d < ND; d++) R.v[k][1 + d] = g[k] >= 0 ? ws[(long long)(C.f0 + d) * G.N + g[k]] : SH(0);
+    for (int d = 0; d < ND; d++)
+      R.v[k][1 + d] = g[k] >= 0 ? at(ws + (long long)(C.f0 + d) * G.N, (unsigned)g[k]) : SH(0);
   }
 }
 
@@ -149,7 +151,7 @@ __device__ inline void stage_q_edge(const Geo& G, const SH* __restrict__ qa, SH*
     }
     SH v[NB];
 #pragma unroll
-    for (int k = 0; k < NB; k++) v[k] = src[k] >= 0 ? qa[src[k]] : (SH)0;
+    for (int k = 0; k < NB; k++) v[k] = src[k] >= 0 ? at(qa, (unsigned)src[k]) : (SH)0;
 #pragma unroll
     for (int k = 0; k < NB; k++)
       if (l0 + 256 * k < L::QN) qb[l0 + 256 * k] = v[k];
@@ -173,7 +175,7 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
   if (ND == 3) inside = inside && o[2] - 3 >= 0 && o[2] + L::TZ + 3 <= G.n[2];
   if (inside) {
     constexpr int PL = L::QX * L::QY, NKF = (PL + 255) / 256;
-    const long long zs = (long long)G.n[0] * G.n[1];
+    const unsigned zs = (unsigned)G.n[0] * (unsigned)G.n[1];
     SH v[NKF][L::QZ], v2[NKF][L::QZ];
 #pragma unroll
     for (int k = 0; k < NKF; k++) {
@@ -182,10 +184,10 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
         const int a = l % L::QX, b = l / L::QX;
         const long long g0 = gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
 #pragma unroll
-        for (int cz = 0; cz < L::QZ; cz++) v[k][cz] = qa[g0 + cz * zs];
+        for (int cz = 0; cz < L::QZ; cz++) v[k][cz] = qa[g0 + cz * (long long)zs];
         if (qa2) {
 #pragma unroll
-          for (int cz = 0; cz < L::QZ; cz++) v2[k][cz] = qa2[g0 + cz * zs];
+          for (int cz = 0; cz < L::QZ; cz++) v2[k][cz] = qa2[g0 + cz * (long long)zs];
         }
       }
     }
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   int x[3] = {o[0] + lc[0], o[1] + lc[1], o[2] + lc[2]};
   const bool valid = interior_c<ND>(G, x);
   const long long N = G.N;
-  const long long c = gix(G, x[0], x[1], x[2]);
+  const unsigned c = cix(G, x[0], x[1], x[2]);  // own cell (used by valid threads only)
   const double* __restrict__ w = A.win;           // own-cell values (FP64)
   const SH* __restrict__ ws = (const SH*)A.wsh;    // neighbour values (stencil precision)
   const double* __restrict__ bg = A.bg;
@@ -254,8 +256,8 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
       dx[d] = valid ? (AT)A.dx1[od + x[d]] : AT(1);
       rdx[d] = valid ? (AT)A.rdx1[od + x[d]] : AT(1);
     } else {
-      dx[d] = valid ? (AT)A.delx[(long long)d * N + c] : AT(1);
-      rdx[d] = valid ? (AT)A.rdelx[(long long)d * N + c] : AT(1);
+      dx[d] = valid ? (AT)at(A.delx + (long long)d * N, c) : AT(1);
+      rdx[d] = valid ? (AT)at(A.rdelx + (long long)d * N, c) : AT(1);
     }
   }
 
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
         const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
         const int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
         in[k] = l < R && in_grid<ND>(G, xr);
-        if (in[k]) load_cell<ND, SH, AT>(G, ws, bg, gix(G, xr[0], xr[1], xr[2]), S[k]);
+        if (in[k]) load_cell<ND, SH, AT>(G, ws, bg, cix(G, xr[0], xr[1], xr[2]), S[k]);
       }
 #pragma unroll
       for (int k = 0; k < 2; k++) {
@@ -326,8 +328,8 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
     }
   }
 
-  const double rho_c = valid ? w[c] : 0.0;
-  const double rhob_c = valid ? bg[(long long)bRHOB() * N + c] : 1.0;
+  const double rho_c = valid ? at(w, c) : 0.0;
+  const double rhob_c = valid ? at(bg + (long long)bRHOB() * N, c) : 1.0;
   const AT rt_c = (AT)(rho_c + rhob_c);
 
   // ------------------------------------------------------------ hyperdiffusion + energy terms
@@ -384,11 +386,11 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
     double bb2 = 0.0, bbd[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      bbd[d] = bg[(long long)bBB(d) * N + c];
+      bbd[d] = at(bg + (long long)bBB(d) * N, c);
       bbc[d] = (AT)bbd[d];
       bb2 = d == 0 ? bbd[d] * bbd[d] : bb2 + bbd[d] * bbd[d];
     }
-    ptb = (AT)((G.gamma - 1) * bg[(long long)bEB() * N + c] - 0.5 * (G.gamma - 2) * bb2);
+    ptb = (AT)((G.gamma - 1) * at(bg + (long long)bEB() * N, c) - 0.5 * (G.gamma - 2) * bb2);
   }
 
   // momentum phases (q = v_f): energy terms, diagonal and off-diagonal sources
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
             const int rel[3] = {bc[0] - ex[0], bc[1] - ex[1], ND == 3 ? bc[2] - ex[2] : 0};
             int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
             const bool in = interior_c<ND>(G, xr);
-            const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
+            const unsigned gq = in ? cix(G, xr[0], xr[1], xr[2]) : 0u;
             const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
             const AT rt = in ? CBX[pi] : AT(0);
             const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
@@ -468,8 +470,8 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
               if (in) {
                 // d != f: the border cell's coordinate along d is a tile coordinate
                 const int ti = (d == 0 ? 0 : (d == 1 ? TX : TX + TY)) + rel[d];
-                const AT dxr = A.dx1 ? SDX[ti] : (AT)A.delx[(long long)d * N + gq];
-                const AT rdxr = A.dx1 ? SDX[L::DXT + ti] : (AT)A.rdelx[(long long)d * N + gq];
+                const AT dxr = A.dx1 ? SDX[ti] : (AT)at(A.delx + (long long)d * N, gq);
+                const AT rdxr = A.dx1 ? SDX[L::DXT + ti] : (AT)at(A.rdelx + (long long)d * N, gq);
                 AT nr, nl;
                 nu_win<SH, AT>(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
                 const AT t5 =
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
           const int rel[3] = {bc[0] - ex[0], bc[1] - ex[1], ND == 3 ? bc[2] - ex[2] : 0};
           int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
           const bool in = interior_c<ND>(G, xr);
-          const long long gq = in ? gix(G, xr[0], xr[1], xr[2]) : 0;
+          const unsigned gq = in ? cix(G, xr[0], xr[1], xr[2]) : 0u;
           const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
           const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
 #pragma unroll
@@ -580,8 +582,8 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
             if (in) {
               // d != f: the border cell's coordinate along d is a tile coordinate
               const int ti = (d == 0 ? 0 : (d == 1 ? TX : TX + TY)) + rel[d];
-              const AT dxr = A.dx1 ? SDX[ti] : (AT)A.delx[(long long)d * N + gq];
-              const AT rdxr = A.dx1 ? SDX[L::DXT + ti] : (AT)A.rdelx[(long long)d * N + gq];
+              const AT dxr = A.dx1 ? SDX[ti] : (AT)at(A.delx + (long long)d * N, gq);
+              const AT rdxr = A.dx1 ? SDX[L::DXT + ti] : (AT)at(A.rdelx + (long long)d * N, gq);
               AT nr, nl;
               nu_win<SH, AT>(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
               const AT t2 =
@@ -627,23 +629,23 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       acc[fMOM(d)] += (AT)(G.g[d] * rho_c);
-      acc[fE(ND)] += (AT)(rho_c * (G.g[d] * w[(long long)fMOM(d) * N + c]) / (rho_c + rhob_c));
+      acc[fE(ND)] += (AT)(rho_c * (G.g[d] * at(w + (long long)fMOM(d) * N, c)) / (rho_c + rhob_c));
     }
   }
   // ------------------------------------------------------------ update, source, RK, ghosts
   const double h = A.hfrac * A.dts[0];
   double out[NP];
 #pragma unroll
-  for (int f = 0; f < NP; f++) out[f] = A.base[(long long)f * N + c] + h * (double)acc[f];
+  for (int f = 0; f < NP; f++) out[f] = at(A.base + (long long)f * N, c) + h * (double)acc[f];
   if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_VERTICAL) {
     const double* dv = G.drv;
-    const double zp = A.pos[c] - dv[2];
-    const double xp = A.pos[N + c] - dv[3];
+    const double zp = at(A.pos, c) - dv[2];
+    const double xp = at(A.pos + N, c) - dv[3];
     const double ez = exp(-zp * zp / (dv[5] * dv[5]));
     const double exx = exp(-xp * xp / (dv[6] * dv[6]));
     double exyz = exx * ez;
     if (ND == 3) {
-      const double yp = A.pos[2 * N + c] - dv[4];
+      const double yp = at(A.pos + 2 * N, c) - dv[4];
       const double ey = exp(-yp * yp / (dv[6] * dv[6]));
       exyz = exx * ey * ez;
     }
@@ -658,37 +660,47 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   if (A.r1) {
 #pragma unroll
     for (int f = 0; f < NP; f++) {
-      const long long q = (long long)f * N + c;
-      out[f] = out[f] + (A.r1[q] + 2.0 * A.r2[q] + A.r3[q] - 4.0 * A.base[q]) / 3;
+      const long long o_f = (long long)f * N;
+      out[f] = out[f] + (at(A.r1 + o_f, c) + 2.0 * at(A.r2 + o_f, c) + at(A.r3 + o_f, c) - 4.0 * at(A.base + o_f, c)) / 3;
     }
   }
   double* __restrict__ wo = A.wout;
-  // ghost images: every combination of this cell's images across the
-  // directions (a fixed direction has none; with n >= 8 an interior
-  // coordinate feeds at most two ghosts per direction), with the asymm signs
+  // the cell itself, then its ghost images: every combination of its images
+  // across the directions (a fixed direction has none; with n >= 8 an
+  // interior coordinate feeds at most two ghosts per direction), with the
+  // asymm signs.  All indices are compile-time so no register array is
+  // indexed at run time; waves away from the domain edge skip the images.
+#pragma unroll
+  for (int q = 0; q < NP; q++) put(wo + (long long)q * N, c, out[q]);
 #pragma unroll
   for (int f = 0; f < NP; f++) {
     if (f > 0 && G.bc_uniform) break;
-    int li[3][3], ln[3] = {1, 1, 1};
+    Img2 I[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     unsigned neg = 0;  // bit d: direction d flips the sign
 #pragma unroll
-    for (int d = 0; d < 3; d++) li[d][0] = x[d];
-#pragma unroll
     for (int d = 0; d < ND; d++) {
-      ln[d] = 1 + bc_images(G.bct[f][d], x[d], G.n[d], ND, &li[d][1], 2);
+      I[d] = bc_images2(G.bct[f][d], x[d], G.n[d], ND);
       if (G.bct[f][d] == BC_ASYMM) neg |= 1u << d;
     }
-    for (int a = 0; a < ln[0]; a++)
-      for (int b2 = 0; b2 < ln[1]; b2++)
-        for (int c2 = 0; c2 < ln[2]; c2++) {
-          const long long cc = gix(G, li[0][a], li[1][b2], li[2][c2]);
+    if (I[0].m + I[1].m + I[2].m == 0) continue;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b2 = 0; b2 < 3; b2++)
+#pragma unroll
+        for (int c2 = 0; c2 < (ND == 3 ? 3 : 1); c2++) {
+          if (a + b2 + c2 == 0 || a > I[0].m || b2 > I[1].m || c2 > I[2].m) continue;
+          const int ga = a == 0 ? x[0] : (a == 1 ? I[0].g0 : I[0].g1);
+          const int gb = b2 == 0 ? x[1] : (b2 == 1 ? I[1].g0 : I[1].g1);
+          const int gc = c2 == 0 ? x[2] : (c2 == 1 ? I[2].g0 : I[2].g1);
+          const unsigned cc = cix(G, ga, gb, gc);
           const unsigned flip = ((a ? 1u : 0u) & neg) ^ ((b2 ? 2u : 0u) & neg) ^ ((c2 ? 4u : 0u) & neg);
           const bool minus = __builtin_popcount(flip) & 1;
           if (G.bc_uniform) {
 #pragma unroll
-            for (int q = 0; q < NP; q++) wo[(long long)q * N + cc] = minus ? -out[q] : out[q];
+            for (int q = 0; q < NP; q++) put(wo + (long long)q * N, cc, minus ? -out[q] : out[q]);
           } else {
-            wo[(long long)f * N + cc] = minus ? -out[f] : out[f];
+            put(wo + (long long)f * N, cc, minus ? -out[f] : out[f]);
           }
         }
   }

@@ -223,17 +223,27 @@ def load_param_file(path: str) -> RunParams:
 
 
 def validate(rp: RunParams) -> Optional[str]:
-    """Returns an error message for parameter sets the engine rejects."""
+    """Returns an error message for parameter sets the engine rejects (the
+    same rules as smaug_create's parameter check, smaug_host.hip)."""
     if rp.ndim not in (2, 3):
         return "ndim must be 2 or 3"
     if any(n < 4 for n in rp.ni):
         return "need at least 4 interior cells per dim"
+    cells = 1
+    for d in range(rp.ndim):
+        cells *= rp.ni[d] + 4
+    if cells >= 1 << 29:
+        return f"{cells} cells: grids of 2^29 cells or more are not supported"
     if rp.divbon:
         return "divbon=1 is not supported (every reference model sets PDIVBON 0)"
-    for d in range(rp.ndim):
-        for bt in (rp.bcu[d], rp.bcl[d]):
-            if bt not in (BC_PERIODIC, BC_CONTCD4):
-                return f"boundary type {bt} in dir {d} is not supported (0 and 4 are)"
+    tables = [list(rp.bcu[: rp.ndim]), list(rp.bcl[: rp.ndim])]
+    tables += [list(t[: rp.ndim]) for t in getattr(rp, "bc_field", {}).values()]
+    for t in tables:
+        for d, bt in enumerate(t):
             if bt == BC_PERIODIC and rp.ndim == 3:
                 return "periodic boundaries are supported in 2D only"
+            if bt in (1, 2):
+                return f"boundary type {bt} in dir {d}: the MPI boundary types need a multi-domain run"
+            if not 0 <= bt <= 7:
+                return f"unknown boundary type {bt} in dir {d}"
     return None

@@ -109,8 +109,10 @@ def test_param_text_rejects_malformed_lines():
 @pytest.mark.parametrize("mutate,needle", [
     (lambda rp: setattr(rp, "divbon", 1), "divbon"),
     (lambda rp: setattr(rp, "bcu", [BC_PERIODIC] * 3), "periodic"),
-    (lambda rp: setattr(rp, "bcu", [6, 4, 4]), "boundary type 6"),
+    (lambda rp: setattr(rp, "bcu", [1, 4, 4]), "boundary type 1"),
+    (lambda rp: setattr(rp, "bcu", [9, 4, 4]), "unknown boundary type 9"),
     (lambda rp: setattr(rp, "ni", [3, 8, 8]), "4 interior"),
+    (lambda rp: setattr(rp, "ni", [1020, 1020, 512]), "2^29"),
 ])
 def test_validate_names_unsupported_settings(mutate, needle):
     from smaug_amd import configs
@@ -118,6 +120,15 @@ def test_validate_names_unsupported_settings(mutate, needle):
     mutate(rp)
     msg = validate(rp)
     assert msg is not None and needle in msg
+
+
+@pytest.mark.parametrize("bt", [BC_CONTCD4, 3, 5, 6, 7])
+def test_validate_accepts_engine_boundary_types(bt):
+    from smaug_amd import configs
+    rp = configs.stratified_params(ni=(8, 8, 8))
+    rp.bcu = [bt, 4, 4]
+    rp.bcl = [4, bt, 4]
+    assert validate(rp) is None
 
 
 def test_engine_refuses_to_run_without_library(monkeypatch, tmp_path):
