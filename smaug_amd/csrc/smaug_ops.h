@@ -39,6 +39,23 @@ __device__ inline float tabs(float x) { return fabsf(x); }
 __device__ inline double tmax(double a, double b) { return fmax(a, b); }
 __device__ inline float tmax(float a, float b) { return fmaxf(a, b); }
 
+// 1/x for a positive normal x: the hardware reciprocal refined by Newton-
+// Raphson steps (two in FP64, one in FP32), within an ulp or two of the
+// correctly rounded quotient in 5 (FP64) or 3 (FP32) instructions instead
+// of the dozen of a full-precision division.
+__device__ inline double rcp_pos(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ inline float rcp_pos(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
+}
+
 template <int ND>
 __device__ inline bool in_grid(const Geo& G, const int* x) {
 #pragma unroll
@@ -119,7 +136,7 @@ __device__ inline void nu7(const T* Q, T dxk, T& nur, T& nul) {
   const T a = m5r > T(0) ? m5r : T(1), b = m5l > T(0) ? m5l : T(1);
   const T p = a * b;
   if (p > lo && p < hi) {
-    const T r = T(1) / p;
+    const T r = rcp_pos(p);
     nur = m5r > T(0) ? dxk * ((m3r * b) * r) : T(0);
     nul = m5l > T(0) ? dxk * ((m3l * a) * r) : T(0);
   } else {
@@ -169,9 +186,9 @@ __device__ inline void fluxes_of(const Geo& G, const CellState<ND, T>& S, int d,
   const T *m = S.m, *b = S.b, *bb = S.bb;
   const T rho = S.rho, rhob = S.rhob, e = S.e, eb = S.eb;
   const T rt = rho + rhob;
-  // one reciprocal for the ND + 1 quotients by rho_t (each then carries
-  // one rounding more than a division)
-  const T rrt = T(1) / rt;
+  // one reciprocal for the ND + 1 quotients by rho_t (rho_t > 0; each
+  // quotient then carries a rounding or two more than a division)
+  const T rrt = rcp_pos(rt);
   T v[ND];
 #pragma unroll
   for (int q = 0; q < ND; q++) v[q] = m[q] * rrt;

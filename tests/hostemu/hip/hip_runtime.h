@@ -127,3 +127,8 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   return hipSuccess;
 }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+
+// the hardware reciprocal: exact on the host (the refinement steps that
+// follow it in the kernels then change nothing)
+#define __builtin_amdgcn_rcp(x) (1.0 / (x))
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
