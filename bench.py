@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # pass B of the two-pass stage): 1664 B per cell-update of an RK4 step minus
 # the wave-speed pass (13 doubles per stage), divided over the 4 stages.
 RHS_BYTES_PER_CELL = (1664 - 4 * 13 * 8) / 4.0
-PRIM_BYTES_PER_CELL = 13 * 8.0
+PRIM_BYTES_PER_CELL = (13 + 4) * 8.0  # k_prim (FP64): reads 13 doubles per cell, writes v and q_e
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rhs_stratified256_traffic.txt")
 
 

@@ -48,8 +48,7 @@ __host__ __device__ inline int bRHOB() { return 1; }
 __host__ __device__ inline int bBB(int d) { return 2 + d; }
 // prim indices
 __host__ __device__ inline int pV(int d) { return d; }
-__host__ __device__ inline int pPT(int nd) { return nd; }
-__host__ __device__ inline int pQE(int nd) { return nd + 1; }
+__host__ __device__ inline int pQE(int nd) { return nd; }
 
 // Boundary-table indices the reference's gradient calls pass as "field"
 // (quirk Q2: derived and temporary arrays index boundtype[] with their own

@@ -258,7 +258,7 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   s->nd = p->ndim;
   s->np = 2 + 2 * s->nd;
   s->nb = 2 + s->nd;
-  s->npr = s->nd + 2;
+  s->npr = s->nd + 1;  // v_0..v_{nd-1}, q_e
   s->nvar = 4 + 3 * s->nd;
   s->N = s->G.N;
   const size_t bytes = sizeof(double) * (size_t)s->N;

@@ -25,8 +25,9 @@ __device__ inline double bits2d(unsigned long long b) { return __longlong_as_dou
 // ---------------------------------------------------------------------------
 // k_prim: pointwise derived fields + wave-speed maxima
 //   v_d  = m_d/(rho+rhob)                         src/dervfields.cu:79-98
-//   p_t  = (g-1)(e - m^2/2rho_t) - (g-2)(b.bb + b^2/2) src/dervfields.cu:107-153
 //   q_e  = e - (b^2 + m^2/rho_t)/2                src/hyperdifesource1.cu:391-395
+// (p_t, src/dervfields.cu:107-153, is formed by k_rhs's flux function from
+// the cell state where it is needed)
 //   c_s, c_fast,d                                 src/dervfields.cu:241-309
 // With SH = float (mixed precision) prim is stored in FP32 and the FP32
 // shadow of the evolved fields (what the stencils of k_rhs read) is written
@@ -40,34 +41,37 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
   const long long N = G.N;
   double mx[3] = {0.0, 0.0, 0.0};
   int bad = 0;
-  const long long stride = (long long)gridDim.x * 256;
-  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += stride) {
-    const double gm = G.gamma;
-    const double rho = w[c], rhob = bg[(long long)bRHOB() * N + c];
-    double m[ND], b[ND], bb[ND];
+  struct In {
+    double rho, rhob, e, eb, m[ND], b[ND], bb[ND];
+  };
+  auto load = [&](long long c, In& q) {
+    q.rho = w[c];
+    q.rhob = bg[(long long)bRHOB() * N + c];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      m[d] = w[(long long)fMOM(d) * N + c];
-      b[d] = w[(long long)fB(ND, d) * N + c];
-      bb[d] = bg[(long long)bBB(d) * N + c];
+      q.m[d] = w[(long long)fMOM(d) * N + c];
+      q.b[d] = w[(long long)fB(ND, d) * N + c];
+      q.bb[d] = bg[(long long)bBB(d) * N + c];
     }
-    const double e = w[(long long)fE(ND) * N + c];
-    const double eb = bg[(long long)bEB() * N + c];
-    const double rt = rho + rhob;
-    double mm = m[0] * m[0], b2 = b[0] * b[0], bbb = b[0] * bb[0], bb2 = bb[0] * bb[0];
+    q.e = w[(long long)fE(ND) * N + c];
+    q.eb = bg[(long long)bEB() * N + c];
+  };
+  auto cell = [&](long long c, const In& q) {
+    const double gm = G.gamma;
+    const double *m = q.m, *b = q.b, *bb = q.bb;
+    const double rho = q.rho, e = q.e, eb = q.eb;
+    const double rt = rho + q.rhob;
+    double mm = m[0] * m[0], b2 = b[0] * b[0], bb2 = bb[0] * bb[0];
     double bbb_c = bb[0] * b[0];
 #pragma unroll
     for (int d = 1; d < ND; d++) {
       mm = mm + m[d] * m[d];
       b2 = b2 + b[d] * b[d];
-      bbb = bbb + b[d] * bb[d];
       bb2 = bb2 + bb[d] * bb[d];
       bbb_c = bbb_c + bb[d] * b[d];
     }
 #pragma unroll
     for (int d = 0; d < ND; d++) prim[(long long)pV(d) * N + c] = (SH)(m[d] / rt);
-    const double v = (gm - 2.0) * (bbb + 0.5 * b2);
-    prim[(long long)pPT(ND) * N + c] = (SH)((gm - 1.0) * (e - 0.5 * mm / rt) - v);
     prim[(long long)pQE(ND) * N + c] = (SH)(e - 0.5 * (b2 + mm / rt));
     if (shadow) {
       shadow[c] = (float)rho;
@@ -92,6 +96,12 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
       double t = cfd > cs ? cfd : cs;
       if (t > mx[d]) mx[d] = t;
     }
+  };
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += stride) {
+    In q0;
+    load(c, q0);
+    cell(c, q0);
   }
   // wavefront (64 lanes) then block reduction of the three maxima
 #pragma unroll
