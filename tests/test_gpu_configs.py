@@ -6,7 +6,13 @@
 * the stratified atmosphere at 128^3 (the 3D FP64 configuration's grid) for
   two RK4 steps -- the oracle needs ~15 s per step there, so the 1000-step
   run is checked on smaller grids (test_gpu_parity.py) and here only at
-  full size.
+  full size;
+* the forward step on the grids of the two adjoint configurations (with the
+  reference's Gaussian driver; the amplitude-map control belongs to the
+  differentiable entry points): the 2D slice at 512 (z) x 256 (x), 5 RK4
+  steps, and 192^3, one RK4 step;
+* mixed precision at 128^3 against the FP64 oracle, two RK4 steps (bar
+  1e-5).
 Bar: relative L2 per evolved variable <= 1e-12 where the configuration's
 roundoff sensitivity allows it (see the Orszag-Tang test).
 """
@@ -84,3 +90,47 @@ def test_stratified_128cubed_two_rk4_steps():
     err = compare(rp, got, ref)
     print("rel L2 per variable:", err, flush=True)
     assert np.all(err <= TOL), err
+
+
+def _forward_vs_oracle(rp, w, pos, nsteps, tol, label):
+    print(f"[config] {label}", flush=True)
+    e = Engine(rp, w, pos)
+    e.step(nsteps)
+    got = e.state()
+    e.close()
+    t0 = time.time()
+    o = oracle.Oracle(rp, w, pos)
+    for k in range(nsteps):
+        o.step(1)
+        print(f"  oracle step {k + 1}, {time.time() - t0:.0f} s", flush=True)
+    ref = o.state()
+    o.close()
+    err = compare(rp, got, ref)
+    print("rel L2 per variable:", err, flush=True)
+    assert np.all(err <= tol), err
+    return err
+
+
+def test_stratified_2d_slice_512x256():
+    """The 2D adjoint configuration's grid: 512 (z) x 256 (x)."""
+    rp = configs.stratified_params(ni=(512, 256), ndim=2, rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    _forward_vs_oracle(rp, w, pos, 5, TOL, "stratified 2D 512x256 RK4, 5 steps")
+
+
+def test_stratified_192cubed_one_step():
+    """The 3D adjoint configuration's grid: 192^3."""
+    rp = configs.stratified_params(ni=(192, 192, 192), rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    _forward_vs_oracle(rp, w, pos, 1, TOL, "stratified 192^3 RK4, 1 step")
+
+
+def test_mixed_precision_stratified_128cubed():
+    """Mixed precision on the 3D configuration's grid against the FP64 oracle."""
+    from smaug_amd.params import PREC_MIXED
+
+    rp = configs.stratified_params(ni=(128, 128, 128), rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    rp.precision = PREC_MIXED
+    err = _forward_vs_oracle(rp, w, pos, 2, 1e-5, "stratified 128^3 mixed precision, 2 steps")
+    assert np.any(err > 0), "mixed run identical to FP64: shadows not in use"
