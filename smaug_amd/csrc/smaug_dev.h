@@ -6,7 +6,7 @@
 // and the static background are separate arrays:
 //   pert  : rho, mom[0..nd), energy, b[0..nd)         (np = 2+2*nd fields)
 //   bg    : energyb, rhob, bb[0..nd)                  (nb = 2+nd fields)
-//   prim  : v[0..nd), p_t, q_e                        (cell-local, per stage)
+//   prim  : v[0..nd), q_e                             (cell-local, per stage)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

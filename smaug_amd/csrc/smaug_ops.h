@@ -180,9 +180,9 @@ __device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const d
 
 template <int ND, typename T = double>
 __device__ inline void fluxes_of(const Geo& G, const CellState<ND, T>& S, int d, T* F) {
-  // v = m/rho_t and p_t are recomputed from the cell's state (4 fewer
-  // loads per flux cell than reading k_prim's copies); p_t follows
-  // computept3, src/dervfields.cu:107-153
+  // v = m/rho_t and p_t are formed from the cell's state (k_prim keeps no
+  // p_t, and forming v here takes fewer loads per flux cell than reading
+  // its copies); p_t follows computept3, src/dervfields.cu:107-153
   const T *m = S.m, *b = S.b, *bb = S.bb;
   const T rho = S.rho, rhob = S.rhob, e = S.e, eb = S.eb;
   const T rt = rho + rhob;
