@@ -280,38 +280,60 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   // ------------------------------------------------------------ fluxes
   if (T & SMAUG_T_FLUX) {
     AT* FB = sm + L::CB;
+    // the thread's own tile cell: its state is loaded once for every
+    // direction (and the flux function's v, p_t, ... of it are shared); a
+    // second slot per direction covers the 2 + 2 halo layers along it
+    const bool in0 = in_grid<ND>(G, x);
+    CellState<ND, AT> S0;
+    if (in0) load_cell<ND, SH, AT>(G, ws, bg, c, S0);
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       const int ex[3] = {d == 0 ? 2 : 0, d == 1 ? 2 : 0, d == 2 ? 2 : 0};
-      const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1], rz = ND == 3 ? TZ + 2 * ex[2] : 1;
-      // at most two region cells per thread: both cells' state loads go
-      // out together, then the fluxes are formed and stored
-      static_assert(L::FN <= 512, "two flux cells per thread");
-      const int R = rx * ry * rz;
-      CellState<ND, AT> S[2];
-      bool in[2];
-#pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const int l = tid + 256 * k;
-        const int a = l % rx, b = (l / rx) % ry, cz = l / (rx * ry);
-        const int xr[3] = {o[0] - ex[0] + a, o[1] - ex[1] + b, ND == 3 ? o[2] - ex[2] + cz : 0};
-        in[k] = l < R && in_grid<ND>(G, xr);
-        if (in[k]) load_cell<ND, SH, AT>(G, ws, bg, cix(G, xr[0], xr[1], xr[2]), S[k]);
+      const int rx = TX + 2 * ex[0], ry = TY + 2 * ex[1];
+      const int Td = d == 0 ? TX : (d == 1 ? TY : TZ);
+      const int pa = d == 0 ? 1 : 0, qa = d == 2 ? 1 : 2;  // the other two axes
+      const int Tp = pa == 0 ? TX : TY;
+      const int Fd = Tp * (ND == 3 ? (qa == 1 ? TY : TZ) : 1);  // face of the tile across d
+      static_assert(ND == 3 ? 4 * TY * TZ <= 256 && 4 * TX * TZ <= 256 && 4 * TX * TY <= 256
+                            : 4 * TX <= 256 && 4 * TY <= 256,
+                    "the halo layers of a direction fit one slot per thread");
+      CellState<ND, AT> S1;
+      bool in1 = false;
+      int l1 = 0;
+      const bool h1 = tid < 4 * Fd;
+      if (h1) {
+        const int layer = tid / Fd, r = tid - layer * Fd;
+        int rc[3] = {0, 0, 0};
+        rc[d] = layer < 2 ? layer : Td + layer;  // region coordinates 0, 1 and Td+2, Td+3
+        rc[pa] = r % Tp;
+        if (ND == 3) rc[qa] = r / Tp;
+        const int xr[3] = {o[0] - ex[0] + rc[0], o[1] - ex[1] + rc[1], ND == 3 ? o[2] - ex[2] + rc[2] : 0};
+        in1 = in_grid<ND>(G, xr);
+        l1 = rc[0] + rx * (rc[1] + ry * rc[2]);
+        if (in1) load_cell<ND, SH, AT>(G, ws, bg, cix(G, xr[0], xr[1], xr[2]), S1);
       }
+      {
+        AT F[NP];
+        if (in0) {
+          fluxes_of<ND, AT>(G, S0, d, F);
+        } else {
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const int l = tid + 256 * k;
-        if (l < R) {
-          AT F[NP];
-          if (in[k]) {
-            fluxes_of<ND, AT>(G, S[k], d, F);
-          } else {
-#pragma unroll
-            for (int f = 0; f < NP; f++) F[f] = AT(0);
-          }
-#pragma unroll
-          for (int f = 0; f < NP; f++) FB[f * L::FN + l] = F[f];
+          for (int f = 0; f < NP; f++) F[f] = AT(0);
         }
+        const int l0 = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+#pragma unroll
+        for (int f = 0; f < NP; f++) FB[f * L::FN + l0] = F[f];
+      }
+      if (h1) {
+        AT F[NP];
+        if (in1) {
+          fluxes_of<ND, AT>(G, S1, d, F);
+        } else {
+#pragma unroll
+          for (int f = 0; f < NP; f++) F[f] = AT(0);
+        }
+#pragma unroll
+        for (int f = 0; f < NP; f++) FB[f * L::FN + l1] = F[f];
       }
       __syncthreads();
       if (valid && !z4(G, zFLUX(ND), d, x[d])) {
