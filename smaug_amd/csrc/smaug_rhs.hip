@@ -224,15 +224,29 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   const int tid = threadIdx.x;
   if (A.zero_next && blockIdx.x == 0 && tid < 3) A.zero_next[tid] = 0ull;
   // XCD-aware tile order: the hardware deals consecutive workgroups to the
-  // 8 XCDs in turn, so workgroup b runs on XCD b%8; give XCD x the
-  // contiguous block of tiles [x*nb/8, (x+1)*nb/8) so neighbouring tiles
-  // share that XCD's L2 for their halos.
+  // 8 XCDs in turn, so workgroup b runs on XCD b%8.  In 3D XCD x takes the
+  // slab of tile layers [x*gz/8, (x+1)*gz/8) and walks it x fastest, then
+  // z, then y, so the z neighbour of a tile runs gx tiles after it (its
+  // halo planes still in that XCD's L2) and the y neighbour gx*gz/8 tiles
+  // after it (measured 1.7 % faster at 256^3 than x, y, z order, whose z
+  // neighbour comes gx*gy tiles later); otherwise XCD x takes the tiles
+  // [x*nb/8, (x+1)*nb/8) in x, y, z order.
   const int gx = (G.n[0] - 4 + TX - 1) / TX, gy = (G.n[1] - 4 + TY - 1) / TY;
   const int gz = ND == 3 ? (G.n[2] - 4 + TZ - 1) / TZ : 1;
   const int nb = gx * gy * gz;
   const int b = blockIdx.x;
   int t = b;
-  if ((nb & 7) == 0) t = (b & 7) * (nb >> 3) + (b >> 3);
+  if ((nb & 7) == 0) {
+    const int xcd = b & 7, tl = b >> 3;
+    if (ND == 3 && (gz & 7) == 0) {
+      const int zc = gz >> 3;  // tile layers per XCD
+      const int ux = tl % gx, r = tl / gx;
+      const int uz = xcd * zc + r % zc, uy = r / zc;
+      t = (uz * gy + uy) * gx + ux;
+    } else {
+      t = xcd * (nb >> 3) + tl;
+    }
+  }
   const int tbx = t % gx, tby = (t / gx) % gy, tbz = t / (gx * gy);
   const int o[3] = {2 + tbx * TX, 2 + tby * TY, ND == 3 ? 2 + tbz * TZ : 0};
   const int lc[3] = {tid % TX, (tid / TX) % TY, ND == 3 ? tid / (TX * TY) : 0};
