@@ -224,6 +224,10 @@ struct RhsArgs {
   double hfrac;                         // stage step = hfrac*dt
   double tdep;                          // driver time factor sin(2 pi qt/P)
   const double* amap;                   // driver amplitude per (j,k) column, or null
+  const double* dshape;                 // vertical driver on a separable grid: its
+                                        // factors exp(-dz^2/sz^2) over dim 0, then
+                                        // exp(-dx^2/sxy^2) over dim 1 and
+                                        // exp(-dy^2/sxy^2) over dim 2 (else null)
 };
 
 // Adjoint of one stage (smaug_adj.hip).  The stage maps its input win to
@@ -286,5 +290,8 @@ void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const dou
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
 void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A);
 void launch_fill(const Geo& G, hipStream_t st, double* w);
+// the per-axis factors of the vertical driver's shape from per-axis
+// positions pos1 (n0 + n1 [+ n2] values) into dshape (same length)
+void launch_drvtab(const Geo& G, hipStream_t st, const double* pos1, double* dshape);
 
 }  // namespace smaug

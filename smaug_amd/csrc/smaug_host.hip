@@ -65,6 +65,7 @@ struct smaug_ctx {
   double* dx1 = nullptr;   // separable grid: per-coordinate spacing (n0+n1+n2)
   double* rdx1 = nullptr;
   double* pos = nullptr;
+  double* dshape = nullptr;  // per-axis factors of the vertical driver (separable grid)
   unsigned long long* cmax = nullptr;  // 3 slots of 4
   double* dts = nullptr;               // dt, t
   int* flag = nullptr;
@@ -238,6 +239,7 @@ static void free_ctx(smaug_t* s) {
   hipFree(s->dx1);
   hipFree(s->rdx1);
   hipFree(s->pos);
+  hipFree(s->dshape);
   hipFree(s->cmax);
   hipFree(s->dts);
   hipFree(s->flag);
@@ -304,6 +306,25 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
       return fail(SMAUG_E_NOMEM, "device allocation failed");
     }
   }
+  // vertical driver on a grid whose positions along d depend on the d
+  // coordinate only: its shape is a product of per-axis factors, formed
+  // once here instead of three exponentials per cell and stage
+  std::vector<double> h_pos1;
+  double* d_pos1 = nullptr;
+  if (p->driver == SMAUG_DRIVER_VERTICAL && separable_spacing(s->nd, p->n, s->h_pos, h_pos1)) {
+    ok &= hipMalloc(&s->dshape, sizeof(double) * h_pos1.size()) == hipSuccess;
+    ok &= hipMalloc(&d_pos1, sizeof(double) * h_pos1.size()) == hipSuccess;
+    ok = ok && hipMemcpy(d_pos1, h_pos1.data(), sizeof(double) * h_pos1.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+      launch_drvtab(s->G, s->stream, d_pos1, s->dshape);
+      ok = hipStreamSynchronize(s->stream) == hipSuccess;
+    }
+    hipFree(d_pos1);
+    if (!ok) {
+      free_ctx(s);
+      return fail(SMAUG_E_NOMEM, "driver table setup failed");
+    }
+  }
   const double dts0[2] = {p->moddton == 1 ? 1.0e-8 : p->dt, 0.0};  // src/iosmaug.c:105,333-336
   const unsigned long long z12[16] = {0};
   const int z = 0;
@@ -359,6 +380,7 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
   A.hfrac = hfrac;
   A.tdep = tdep;
   A.amap = s->amap;
+  A.dshape = s->dshape;
   return A;
 }
 

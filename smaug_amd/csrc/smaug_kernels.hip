@@ -225,6 +225,30 @@ void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const dou
 
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A) { launch_rhs_tiled(G, st, A); }
 
+// per-axis factors of the vertical driver (models/usersource.cu.fluxtube1:3-90):
+// the same expressions k_rhs evaluates per cell when the grid is not separable
+__global__ __launch_bounds__(256) void k_drvtab(Geo G, const double* __restrict__ pos1, double* __restrict__ dshape) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int n0 = G.n[0], n1 = G.n[1], n2 = G.nd == 3 ? G.n[2] : 0;
+  if (t >= n0 + n1 + n2) return;
+  const double* dv = G.drv;
+  if (t < n0) {
+    const double zp = pos1[t] - dv[2];
+    dshape[t] = exp(-zp * zp / (dv[5] * dv[5]));
+  } else if (t < n0 + n1) {
+    const double xp = pos1[t] - dv[3];
+    dshape[t] = exp(-xp * xp / (dv[6] * dv[6]));
+  } else {
+    const double yp = pos1[t] - dv[4];
+    dshape[t] = exp(-yp * yp / (dv[6] * dv[6]));
+  }
+}
+
+void launch_drvtab(const Geo& G, hipStream_t st, const double* pos1, double* dshape) {
+  const int n = G.n[0] + G.n[1] + (G.nd == 3 ? G.n[2] : 0);
+  hipLaunchKernelGGL(k_drvtab, dim3((n + 255) / 256), dim3(256), 0, st, G, pos1, dshape);
+}
+
 void launch_fill(const Geo& G, hipStream_t st, double* w) {
   const unsigned nb = (unsigned)((G.N + 255) / 256);
   if (G.nd == 3)

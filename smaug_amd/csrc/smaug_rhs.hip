@@ -691,15 +691,23 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   for (int f = 0; f < NP; f++) out[f] = at(A.base + (long long)f * N, c) + h * (double)acc[f];
   if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_VERTICAL) {
     const double* dv = G.drv;
-    const double zp = at(A.pos, c) - dv[2];
-    const double xp = at(A.pos + N, c) - dv[3];
-    const double ez = exp(-zp * zp / (dv[5] * dv[5]));
-    const double exx = exp(-xp * xp / (dv[6] * dv[6]));
-    double exyz = exx * ez;
-    if (ND == 3) {
-      const double yp = at(A.pos + 2 * N, c) - dv[4];
-      const double ey = exp(-yp * yp / (dv[6] * dv[6]));
-      exyz = exx * ey * ez;
+    double ez, exyz;
+    if (A.dshape) {  // separable grid: per-axis factors formed at set-up
+      ez = A.dshape[x[0]];
+      const double exx = A.dshape[G.n[0] + x[1]];
+      exyz = exx * ez;
+      if (ND == 3) exyz = exx * A.dshape[G.n[0] + G.n[1] + x[2]] * ez;
+    } else {
+      const double zp = at(A.pos, c) - dv[2];
+      const double xp = at(A.pos + N, c) - dv[3];
+      ez = exp(-zp * zp / (dv[5] * dv[5]));
+      const double exx = exp(-xp * xp / (dv[6] * dv[6]));
+      exyz = exx * ez;
+      if (ND == 3) {
+        const double yp = at(A.pos + 2 * N, c) - dv[4];
+        const double ey = exp(-yp * yp / (dv[6] * dv[6]));
+        exyz = exx * ey * ez;
+      }
     }
     // amplitude map: one amplitude per (j,k) column in place of
     // A exp(-dx^2/sxy^2 [- dy^2/sxy^2])

@@ -144,3 +144,22 @@ def test_nonseparable_grid():
     _, d1 = eng.grid()
     eng.close()
     assert not np.allclose(d1[0].reshape(rp.n[1], rp.n[0])[3], d1[0].reshape(rp.n[1], rp.n[0])[7])
+
+
+def test_nonseparable_grid_driver():
+    """Stratified 3D with the vertical driver on positions that are not a
+    tensor product (dim 1 stretched with height): the stage kernel forms the
+    driver shape per cell instead of from per-axis factors."""
+    rp, w, pos = strat_case(ni=(14, 10, 12), ndim=3, rkon=1)
+    pos = pos.copy()
+    z = pos[0]
+    pos[1] = pos[1] * (1.0 + 0.02 * np.sin(2 * np.pi * (z - z.min()) / (z.max() - z.min())))
+    a, b, ca, cb = run_pair(rp, w, pos, 2)
+    err = compare(rp, a, b)
+    print(f"strat3d non-separable grid with driver RK4 x2 rel_l2={err}", flush=True)
+    assert np.all(err <= TOL), err
+    eng = Engine(rp, w, pos)
+    p1, _ = eng.grid()
+    eng.close()
+    col = p1[1].reshape(rp.n[2], rp.n[1], rp.n[0])[4, 5]
+    assert np.ptp(col) > 0  # position along dim 1 varies along dim 0
