@@ -680,6 +680,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   if (T & SMAUG_T_GRAV) {
 #pragma unroll
     for (int d = 0; d < ND; d++) {
+      if (G.g[d] == 0.0) continue;  // no gravity along d: both terms are zero
       acc[fMOM(d)] += (AT)(G.g[d] * rho_c);
       acc[fE(ND)] += (AT)(rho_c * (G.g[d] * at(w + (long long)fMOM(d) * N, c)) / (rho_c + rhob_c));
     }
