@@ -84,6 +84,36 @@ struct CellBox {
   SH* CBX;  // null: no cell box with this q box
   bool with_rho;
   int f0;
+  bool shell = false;  // 3D: the tile's own cells are already in the box
+                       // (written from the flux phase's cell state); only
+                       // the 1-cell shell around the tile is loaded
+};
+
+// cell l of the 1-cell shell of the 3D cell box (l < SHN): both z faces,
+// then both y faces between them, then both x faces between those
+template <int ND>
+struct Shell {
+  using L = Lay<ND>;
+  static constexpr int ZF = L::PX * L::PY, YF = L::PX * (L::PZ - 2), XF = (L::PY - 2) * (L::PZ - 2);
+  static constexpr int SHN = 2 * (ZF + YF + XF);
+  __device__ static inline void cell(int l, int& a, int& b, int& c) {
+    if (l < 2 * ZF) {
+      const int side = l >= ZF ? 1 : 0, r = l - side * ZF;
+      a = r % L::PX;
+      b = r / L::PX;
+      c = side ? L::PZ - 1 : 0;
+    } else if (l < 2 * (ZF + YF)) {
+      const int r0 = l - 2 * ZF, side = r0 >= YF ? 1 : 0, r = r0 - side * YF;
+      a = r % L::PX;
+      b = side ? L::PY - 1 : 0;
+      c = 1 + r / L::PX;
+    } else {
+      const int r0 = l - 2 * (ZF + YF), side = r0 >= XF ? 1 : 0, r = r0 - side * XF;
+      a = side ? L::PX - 1 : 0;
+      b = 1 + r % (L::PY - 2);
+      c = 1 + r / (L::PY - 2);
+    }
+  }
 };
 
 template <int ND, typename SH>
@@ -101,9 +131,10 @@ __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const
 #pragma unroll
   for (int k = 0; k < NK; k++) {
     const int l = threadIdx.x + 256 * k;
-    const int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
+    int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
+    if (ND == 3 && C.shell && l < Shell<ND>::SHN) Shell<ND>::cell(l, a, b, cz);
     const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
-    g[k] = l < L::PN && in_grid<ND>(G, x) ? (int)cix(G, x[0], x[1], x[2]) : -1;
+    g[k] = l < (ND == 3 && C.shell ? Shell<ND>::SHN : L::PN) && in_grid<ND>(G, x) ? (int)cix(G, x[0], x[1], x[2]) : -1;
   }
 #pragma unroll
   for (int k = 0; k < NK; k++) {
@@ -120,7 +151,14 @@ __device__ inline void cells_store(const CellBox<ND, SH>& C, const CellLoads<ND,
   using L = Lay<ND>;
 #pragma unroll
   for (int k = 0; k < CellLoads<ND, SH>::NK; k++) {
-    const int l = threadIdx.x + 256 * k;
+    const int l0 = threadIdx.x + 256 * k;
+    int l = l0;
+    if (ND == 3 && C.shell) {
+      if (l0 >= Shell<ND>::SHN) continue;
+      int a, b, cz;
+      Shell<ND>::cell(l0, a, b, cz);
+      l = a + L::PX * (b + L::PY * cz);
+    }
     if (l < L::PN) {
       if (C.with_rho) C.CBX[l] = R.v[k][0];
 #pragma unroll
@@ -291,6 +329,10 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
   const bool want_mom = G.hyper && (T & SMAUG_T_HMOM);
   const bool want_eterm = (T & SMAUG_T_ETERM) != 0;
 
+  // FP64 3D: the flux phase's own-cell state fills the tile's cells of the
+  // momentum phases' cell box (the same rho + rho_b sum as the staging), so
+  // that staging loads only the box's shell
+  const bool own_cb = sizeof(SH) == 8 && ND == 3 && want_mom && (T & SMAUG_T_FLUX);
   // ------------------------------------------------------------ fluxes
   if (T & SMAUG_T_FLUX) {
     AT* FB = sm + L::CB;
@@ -300,6 +342,11 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
     const bool in0 = in_grid<ND>(G, x);
     CellState<ND, AT> S0;
     if (in0) load_cell<ND, SH, AT>(G, ws, bg, c, S0);
+    if (own_cb) {  // the tile's cells of the momentum phases' cell box: rho_t, m
+      CBX[pc] = in0 ? S0.rho + S0.rhob : AT(0);
+#pragma unroll
+      for (int q = 0; q < ND; q++) CBX[(1 + q) * L::PN + pc] = in0 ? S0.m[q] : AT(0);
+    }
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       const int ex[3] = {d == 0 ? 2 : 0, d == 1 ? 2 : 0, d == 2 ? 2 : 0};
@@ -435,7 +482,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo 
     for (int f = 0; f < ND; f++) {
       // the first box brings rho_t and m on the cell box with it
       stage_q<ND, SH>(G, pr + (long long)pV(f) * N, qb, o, ws, bg,
-                      CellBox<ND, SH>{f == 0 && want_mom ? CBX : nullptr, true, fMOM(0)});
+                      CellBox<ND, SH>{f == 0 && want_mom ? CBX : nullptr, true, fMOM(0), own_cb});
       __syncthreads();
       if (want_eterm && valid) {  // addenergyterms_cd2 for dir f
         AT gf[ND];
