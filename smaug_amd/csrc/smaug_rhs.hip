@@ -249,9 +249,9 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
   if (qa2) stage_q_edge<ND, SH>(G, qa2, qb2, o);
 }
 
-// FP32 stencils: at least five wavefronts per SIMD (96 VGPRs and a 12-byte
-// spill; measured faster at 256^3 than four without a spill or six with a
-// 72-byte spill).  FP64: three (151 VGPRs, 52 KB of LDS per workgroup).
+// FP32 stencils: at least five wavefronts per SIMD (96 VGPRs; measured
+// faster at 256^3 than four without a spill or six with a 72-byte spill).
+// FP64: three (148 VGPRs, 52 KB of LDS per workgroup).
 template <int ND, typename SH>
 __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
