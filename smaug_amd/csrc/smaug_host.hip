@@ -314,15 +314,21 @@ extern "C" int smaug_create(const smaug_params* p, const double* w0, const doubl
   if (p->driver == SMAUG_DRIVER_VERTICAL && separable_spacing(s->nd, p->n, s->h_pos, h_pos1)) {
     ok &= hipMalloc(&s->dshape, sizeof(double) * h_pos1.size()) == hipSuccess;
     ok &= hipMalloc(&d_pos1, sizeof(double) * h_pos1.size()) == hipSuccess;
-    ok = ok && hipMemcpy(d_pos1, h_pos1.data(), sizeof(double) * h_pos1.size(), hipMemcpyHostToDevice) == hipSuccess;
-    if (ok) {
+    if (!ok) {
+      hipFree(d_pos1);
+      free_ctx(s);
+      return fail(SMAUG_E_NOMEM, "device allocation failed");
+    }
+    hipError_t te = hipMemcpy(d_pos1, h_pos1.data(), sizeof(double) * h_pos1.size(), hipMemcpyHostToDevice);
+    if (te == hipSuccess) {
       launch_drvtab(s->G, s->stream, d_pos1, s->dshape);
-      ok = hipStreamSynchronize(s->stream) == hipSuccess;
+      te = hipGetLastError();
+      if (te == hipSuccess) te = hipStreamSynchronize(s->stream);
     }
     hipFree(d_pos1);
-    if (!ok) {
+    if (te != hipSuccess) {
       free_ctx(s);
-      return fail(SMAUG_E_NOMEM, "driver table setup failed");
+      return fail(SMAUG_E_HIP, "driver shape tables: %s", hipGetErrorString(te));
     }
   }
   const double dts0[2] = {p->moddton == 1 ? 1.0e-8 : p->dt, 0.0};  // src/iosmaug.c:105,333-336
