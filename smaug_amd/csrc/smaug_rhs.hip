@@ -251,9 +251,11 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
 
 // FP32 stencils: at least five wavefronts per SIMD (96 VGPRs; measured
 // faster at 256^3 than four without a spill or six with a 72-byte spill).
-// FP64: three (148 VGPRs, 52 KB of LDS per workgroup).
+// FP64: three, the number its 52 KB of LDS per workgroup allows (146
+// VGPRs); stating it lets the scheduler spend registers up to that bound
+// (measured 0.8 % faster than a bound of one workgroup).
 template <int ND, typename SH>
-__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 1) void k_rhs_tiled(Geo G, RhsArgs A) {
+__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
   using AT = SH;  // arithmetic type of the stencils (the own-cell update and the RK combination are FP64)
   constexpr int NP = L::NP;
