@@ -170,9 +170,14 @@ int smaug_out_write(const char* path, const char* name, int ndim, const int* n, 
 int smaug_write_out(smaug_t* s, const char* prefix, const char* name);
 
 /* ---- driver plug-in, streams, checkpoints --------------------------------
- *   smaug_set_driver_map  <- the amplitude of the vertical-velocity driver
- *                            (models/usersource.cu.fluxtube1:3-90, installed
- *                            as addsourceterms2, src/centdiff2.cu:790-872),
+ *   smaug_set_driver_map  <- the amplitude of the vertical-velocity driver,
+ *                            installed as addsourceterms2 (src/centdiff2.cu:
+ *                            790-872).  The driver is a hybrid of two model
+ *                            files: the vertical forcing m_0 += dt v_z rho_t
+ *                            of models/usersource.cu.spicule1:131, with the
+ *                            3D Gaussian shape (pos1..3 from wd) and the
+ *                            energy term e += dt v^2 rho_t/2 of
+ *                            models/usersource.cu.fluxtube1:40-79,
  *                            given per (j,k) column: n1*n2 values, column
  *                            (j,k) at k*n1+j, ghost columns included.  It
  *                            replaces A exp(-dx^2/sxy^2 [- dy^2/sxy^2]):

@@ -1200,7 +1200,9 @@ static void stage(orc_t* o, int so, int sn, double dt) {
   if (T & SMAUG_T_SOURCE) k_source(o, sn);
 }
 
-/* moddton block, src/iosmaug.c:541-616 (Q4) */
+/* moddton block, src/iosmaug.c:541-616 (Q4): the step restarts from DT
+ * (dt=DT; p->dt=DT, :549-550) and takes the CFL step when it lies in
+ * (SMALLDOUBLE, DT) (:568-569) */
 static void cfl(orc_t* o, int s) {
   double cm = 0;
   for (int dim = 0; dim < o->nd; dim++) cm = k_cmax(o, s, dim);
@@ -1209,6 +1211,7 @@ static void cfl(orc_t* o, int s) {
     double t = cm / (o->p.dx[dim]);
     if (t > courantmax) courantmax = t;
   }
+  o->dt = o->p.dt;
   double dtt = (o->p.courant) / courantmax;
   if (dtt > SMALLD && dtt < o->p.dt) o->dt = dtt;
 }

@@ -285,6 +285,8 @@ void launch_meta(hipStream_t st, double* meta, const double* dts, double qt, dou
 void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg, void* prim, float* shadow,
                  unsigned long long* cmax, int* flag);
 void launch_zero3(hipStream_t st, unsigned long long* p);
+// the blow-up flag k_prim raises, back to 0 (a new state replaces the old)
+void launch_clear_flag(hipStream_t st, int* flag);
 void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,
                double dtcap, int moddton, double* dts);
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);

@@ -485,12 +485,13 @@ extern "C" int smaug_cfl(smaug_t* s, double* cmax_per_dim, double* dt) {
     for (int d = 0; d < s->nd; d++) cmax_per_dim[d] = cm[d];
   if (dt) {
     double v = dts[0];
-    if (s->p.moddton == 1) {
+    if (s->p.moddton == 1) {  // from DT, then the CFL step if smaller (src/iosmaug.c:549-569)
       double courantmax = 1.0e-12;
       for (int d = 0; d < s->nd; d++) {
         double t = cm[s->nd - 1] / s->p.dx[d];
         if (t > courantmax) courantmax = t;
       }
+      v = s->p.dt;
       double dtt = s->p.courant / courantmax;
       if (dtt > 1.0e-12 && dtt < s->p.dt) v = dtt;
     }
@@ -533,6 +534,8 @@ extern "C" int smaug_set_state(smaug_t* s, const double* w) {
   const size_t bytes = sizeof(double) * (size_t)s->N;
   HIPCHK(hipMemcpyAsync(s->U[s->cur], w, bytes * s->np, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(s->bg, w + (size_t)s->np * s->N, bytes * s->nb, hipMemcpyHostToDevice, s->stream));
+  launch_clear_flag(s->stream, s->flag);  // a blow-up of the old state no longer applies
+  HIPCHK(hipGetLastError());
   int rc = copy_fixed_ghosts(s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -607,6 +610,7 @@ extern "C" int smaug_load_pert(smaug_t* s, const double* src) {
   HIPCHK(hipMemcpyAsync(s->U[s->cur], src, sizeof(double) * (size_t)s->np * s->N, hipMemcpyDeviceToDevice,
                         s->stream));
   launch_fill(s->G, s->stream, s->U[s->cur]);
+  launch_clear_flag(s->stream, s->flag);  // a blow-up of the old state no longer applies
   HIPCHK(hipGetLastError());
   return copy_fixed_ghosts(s);
 }

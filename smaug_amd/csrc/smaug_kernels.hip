@@ -133,9 +133,15 @@ __global__ void k_zero3(unsigned long long* p) {
   if (threadIdx.x < 3) p[threadIdx.x] = 0ull;
 }
 
+__global__ void k_clear_flag(int* flag) {
+  if (threadIdx.x == 0) *flag = 0;
+}
+
 // CFL step (src/iosmaug.c:541-616, quirk Q4): the last direction's maximum
-// over every dir's dx; the step keeps its previous value unless the CFL step
-// lies in (1e-12, DT).  dts[0] = dt, dts[1] = accumulated time.
+// over every dir's dx.  With moddton=1 each step starts from DT (dt=DT,
+// p->dt=DT, :549-550) and takes the CFL step when it lies in (1e-12, DT)
+// (:568-569); otherwise the step keeps its value.  dts[0] = dt, dts[1] =
+// accumulated time.
 __global__ void k_dt(const unsigned long long* cmax_bits, int nd, double dx0, double dx1, double dx2,
                      double courant, double dtcap, int moddton, double* dts) {
   if (threadIdx.x != 0) return;
@@ -148,6 +154,7 @@ __global__ void k_dt(const unsigned long long* cmax_bits, int nd, double dx0, do
       double t = cm / dxs[d];
       if (t > courantmax) courantmax = t;
     }
+    dt = dtcap;
     const double dtt = courant / courantmax;
     if (dtt > 1.0e-12 && dtt < dtcap) dt = dtt;
   }
@@ -216,6 +223,8 @@ void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg
 void launch_zero3(hipStream_t st, unsigned long long* p) {
   hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, st, p);
 }
+
+void launch_clear_flag(hipStream_t st, int* flag) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, flag); }
 
 void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,
                double dtcap, int moddton, double* dts) {

@@ -235,9 +235,44 @@ def check_autograd(rkon: int):
     return out
 
 
+def check_blowup(kind: str):
+    """A NaN in one cell's energy, or a negative total density, is reported
+    as SMAUG_E_NUMERIC by the next synchronising call; a clean state loaded
+    afterwards steps without the error."""
+    from smaug_amd.engine import SmaugError
+
+    rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
+    nd = rp.ndim
+    bad = w.copy()
+    c = (6 * rp.n[0]) + 7  # an interior cell
+    if kind == "nan":
+        bad[1 + nd, c] = np.nan
+    else:  # rho + rho_b < 0
+        bad[0, c] = -2.0 * w[3 + 2 * nd, c]
+    e = Engine(rp, bad, pos)
+    e.step(1)
+    out = {"kind": kind}
+    try:
+        e.sync()
+        out["raised"] = None
+    except SmaugError as err:
+        out["raised"] = str(err)
+    e.set_state(w)
+    e.step(1)
+    try:
+        e.sync()
+        out["after_reset"] = "ok"
+    except SmaugError as err:
+        out["after_reset"] = str(err)
+    e.close()
+    return out
+
+
 def main(argv):
     what = argv[0]
-    if what == "fwd":
+    if what == "blowup":
+        r = check_blowup(argv[1])
+    elif what == "fwd":
         r = check_fwd(argv[1])
     elif what == "mixed":
         r = check_mixed(argv[1])

@@ -109,6 +109,23 @@ def test_cfl_matches_oracle():
     assert dt1 == dt2
 
 
+@pytest.mark.parametrize("rkon", [0, 1])
+def test_cfl_step_restarts_from_dt(rkon):
+    """moddton=1 with DT below the CFL limit: every step takes dt = DT
+    (src/iosmaug.c:549-569), so t = n*DT exactly; the states match."""
+    rp, w, pos = ot_case(ni=16, rkon=rkon)
+    orc = oracle.Oracle(rp, w, pos)
+    _, dt_cfl = orc.cfl()
+    orc.close()
+    rp.dt = dt_cfl / 7.0
+    a, b, ca, cb = run_pair(rp, w, pos, 4)
+    print(f"OT rkon={rkon} DT below CFL: clock hip={ca} ref={cb}", flush=True)
+    assert ca[3] == rp.dt and cb[3] == rp.dt
+    assert ca[1] == cb[1] and ca[2] == cb[2]
+    err = compare(rp, a, b)
+    assert np.all(err <= TOL), err
+
+
 def test_strat3d_many_interior_tiles():
     """A grid large enough that most workgroups take the interior fast
     paths (no halo substitutions, no boundary zeroing) and the tile order

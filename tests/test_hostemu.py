@@ -60,6 +60,16 @@ def test_forward_matches_oracle(hostlib, case, align):
     assert max(r["rel_l2"]) <= 1e-12, r
 
 
+@pytest.mark.parametrize("kind", ["nan", "negative_rho_t"])
+def test_blowup_is_reported_and_cleared(hostlib, kind):
+    """k_prim's non-finite wave-speed flag reaches the caller as
+    SMAUG_E_NUMERIC (-4) on the next synchronising call, and a state set
+    afterwards clears it."""
+    r = run(hostlib, "blowup", kind)
+    assert r["raised"] is not None and "(-4)" in r["raised"], r
+    assert r["after_reset"] == "ok", r
+
+
 @pytest.mark.parametrize("case", ["s3e", "s3r", "s2r", "ot"])
 def test_mixed_precision_matches_oracle(hostlib, case):
     """The FP32 stencil path (shadows, LDS boxes and arithmetic) stays finite

@@ -416,6 +416,37 @@ def test_cfl_matches_textbook_fast_speed():
     assert dt == pytest.approx(min(rp.dt, rp.courant * min(rp.dx) / cm[-1]), rel=1e-14)
 
 
+@pytest.mark.parametrize("rkon", [0, 1])
+def test_cfl_step_restarts_from_dt_each_step(rkon):
+    """With moddton=1 every step starts from dt = DT (src/iosmaug.c:549-550)
+    and takes the CFL step only when it is below DT (:568-569): a DT below
+    the CFL limit gives dt == DT on every step and t == n*DT exactly (the
+    initial 1e-8 of :333-336 is never used), a DT above it gives the CFL
+    step of the state at the start of each step."""
+    rp, w, pos = ot_case(ni=16, rkon=rkon)
+    o = Oracle(rp, w, pos)
+    _, dt_cfl = o.cfl()
+    o.close()
+    assert dt_cfl < rp.dt  # the OT configuration's DT (1.0) is a cap only
+    small = dt_cfl / 7.0
+    rp.dt = small
+    o = Oracle(rp, w, pos)
+    t = 0.0
+    for n in range(1, 4):
+        o.step(1)
+        it, tt, qt, dt = o.clock()
+        t += small
+        assert dt == small and tt == t and qt == t, (n, dt, tt, qt)
+    o.close()
+    rp.dt = 1.0
+    o = Oracle(rp, w, pos)
+    for n in range(3):
+        _, want = o.cfl()
+        o.step(1)
+        assert o.clock()[3] == want and want < 1.0
+    o.close()
+
+
 # ---------------------------------------------------------------- source terms
 
 def test_gravity_closed_form():
