@@ -90,7 +90,7 @@ typedef struct smaug_params {
   int rkon;                 /* PRKON: 0 single stage, 1 four-step RK         */
   int moddton;              /* PMODDTON: 1 = CFL step from the wave speeds   */
   int hyperdifmom;          /* PHYPERDIFMOM                                  */
-  int divbon;               /* PDIVBON (must be 0 here)                      */
+  int divbon;               /* PDIVBON: 1 calls the divB source term         */
   double chyp[SMAUG_MAXVAR];/* PCHYP* per field                              */
   double xmin[3], xmax[3];  /* domain (interior)                             */
   double dx[3];             /* p->dx = (xmax-xmin)/NI, used by the CFL rule  */
@@ -105,6 +105,9 @@ typedef struct smaug_params {
   int precision;            /* SMAUG_PREC_*                                  */
   double qt0;               /* initial value of the nominal clock p->qt      */
   int it0;                  /* initial step counter p->it                    */
+  int divbfix;              /* PDIVBFIX: with divbon=1, nonzero applies the
+                               divB source terms (src/divb.cu:165-338); the
+                               reference's kernel acts in 2D only            */
 } smaug_params;
 
 typedef struct smaug_ctx smaug_t;

@@ -469,6 +469,69 @@ static void k_grav(orc_t* o, int so, int sn) {
   }
 }
 
+/* ---------------- divB source terms ----------------------------------- */
+
+/* grad3d_MODID, src/gradops.cu:130-260: the 4th-order central difference
+ * over 2*p->dx (not delx), zeroed at the first/last two interior layers
+ * with the (A || B && C) precedence whatever the boundary types. */
+static double g3d(orc_t* o, const double* a, int i, int j, int k, int dir) {
+  const int x = coord(i, j, k, dir), nn = nof(o, dir);
+  const long c = enc(o, i, j, k), s = stride(o, dir);
+  double g = 0;
+  if (x > 1 && x < nn - 2)
+    g = ((8 * a[c + s] - 8 * a[c - s] + a[c - 2 * s] - a[c + 2 * s]) / 6.0) / (2.0 * o->p.dx[dir]);
+  int t;
+  if (o->nd == 3) {
+    if (dir == 0) t = inr(j, 1, o->n1 - 2) && inr(k, 1, o->n2 - 2);
+    else if (dir == 1) t = inr(i, 1, o->n0 - 2) && inr(k, 1, o->n2 - 2);
+    else t = inr(i, 1, o->n0 - 2) && inr(j, 1, o->n1 - 2);
+  } else {
+    t = dir == 0 ? inr(j, 1, o->n1 - 2) : inr(i, 1, o->n0 - 2);
+  }
+  if (x == nn - 3 || (x == nn - 4 && t)) g = 0;
+  else if (x == 2 || (x == 3 && t)) g = 0;
+  return g;
+}
+
+/* cudivb -> divb_parallel, src/divb.cu:165-338, called after cugrav
+ * (src/iosmaug.c:678-679) when divbon == 1; it acts when divbfix != 0.
+ * In 2D, on cells 2 < i,j < n-2: divb = d1 b1 + d2 b2 + d1 b1b + d2 b2b
+ * (grad3d), dwn1 = 0 then the sources dbsource* (src/divb.cu:17-100):
+ * m_d: -divb b_d, b_d: -divb m_d/rho_t (NaN -> 0), e: -divb wd[bdotv]
+ * with wd[bdotv] never computed (zeroed by the order-0 pass), rho: 0; then
+ * w[ordero] -= dt dwn1 on 1 < i,j < n-2.  The launch covers n0*n1 threads
+ * with k = 0 only and its 3D region test needs k > 2, so in 3D the kernel
+ * changes nothing. */
+static void k_divb(orc_t* o, int so, int sn, double dt) {
+  if (o->p.divbon != 1 || !o->p.divbfix || o->nd != 2) return;
+  const int nvf = F_B(o->nd - 1) + 1;
+  double* divb = WD(o, 2 * o->nd + 7);
+  double* bdotv = WD(o, 2 * o->nd + 6);
+  FOR_ALL(o) {
+    const long c = enc(o, i, j, k);
+    for (int f = F_RHO; f < nvf; f++) o->dwn1[(long)f * o->N + c] = 0;
+  }
+  FOR_ALL(o) {
+    if (!(i > 2 && j > 2 && i < o->n0 - 2 && j < o->n1 - 2)) continue;
+    const long c = enc(o, i, j, k);
+    divb[c] = g3d(o, W(o, so, F_B(0)), i, j, k, 0) + g3d(o, W(o, so, F_B(1)), i, j, k, 1);
+    divb[c] += g3d(o, W(o, so, F_BB(0)), i, j, k, 0) + g3d(o, W(o, so, F_BB(1)), i, j, k, 1);
+    const double rho_t = W(o, so, F_RHO)[c] + W(o, so, F_RHOB)[c];
+    for (int d = 0; d < 2; d++) {
+      double sm = -divb[c] * W(o, so, F_B(d))[c];
+      double sb = -divb[c] * W(o, so, F_MOM(d))[c] / rho_t;
+      o->dwn1[(long)F_MOM(d) * o->N + c] += isnan(sm) ? 0 : sm;
+      o->dwn1[(long)F_B(d) * o->N + c] += isnan(sb) ? 0 : sb;
+    }
+    o->dwn1[(long)F_E * o->N + c] += -divb[c] * bdotv[c];
+  }
+  FOR_ALL(o) {
+    if (!(i > 1 && j > 1 && i < o->n0 - 2 && j < o->n1 - 2)) continue;
+    const long c = enc(o, i, j, k);
+    for (int f = F_RHO; f < nvf; f++) W(o, sn, f)[c] = W(o, sn, f)[c] - dt * o->dwn1[(long)f * o->N + c];
+  }
+}
+
 /* ---------------- hyperdiffusion coefficient -------------------------- */
 
 /* q of field (hyperdifvisc1r_parallel phase 2, src/hyperdifvisc1r.cu:1225-1253) */
@@ -1143,6 +1206,7 @@ static void stage(orc_t* o, int so, int sn, double dt) {
     }
   }
   if (T & SMAUG_T_GRAV) k_grav(o, so, sn);
+  k_divb(o, so, sn, dt); /* src/iosmaug.c:678-679 */
   if (o->p.hyperdifmom == 1) {
     for (int dim = 0; dim < nd; dim++) {
       o->cmax_dim[dim] = k_cmax(o, so, dim);

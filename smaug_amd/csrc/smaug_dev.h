@@ -35,6 +35,8 @@ struct Geo {
   int driver;
   double drv[8];
   int mixed;             // 1: stencils read FP32 shadows (SMAUG_PREC_MIXED)
+  int divb;              // 1: divB source terms (divbon=1, divbfix != 0, 2D)
+  double pdx[3];         // p->dx = (xmax-xmin)/NI (the divB gradients' spacing)
 };
 
 // evolved-field indices

@@ -152,7 +152,7 @@ static int check_params(const smaug_params* p) {
     if (cells >= (1ll << 29))
       return fail(SMAUG_E_UNSUPPORTED, "%lld cells: grids of 2^29 cells or more are not supported", cells);
   }
-  if (p->divbon) return fail(SMAUG_E_UNSUPPORTED, "divbon=1 is not supported");
+  if (p->divbon != 0 && p->divbon != 1) return fail(SMAUG_E_ARG, "divbon must be 0 or 1");
   if (p->rkon != 0 && p->rkon != 1) return fail(SMAUG_E_ARG, "rkon must be 0 or 1");
   if (p->precision != SMAUG_PREC_FP64 && p->precision != SMAUG_PREC_MIXED)
     return fail(SMAUG_E_ARG, "unknown precision");
@@ -207,6 +207,10 @@ static void make_geo(const smaug_params* p, Geo& G) {
   G.driver = p->driver;
   for (int i = 0; i < 8; i++) G.drv[i] = p->drv[i];
   G.mixed = p->precision == SMAUG_PREC_MIXED;
+  // src/divb.cu:165-338 acts in 2D only (its launch covers the k = 0 plane,
+  // which its 3D region test excludes)
+  G.divb = p->divbon == 1 && p->divbfix != 0 && p->ndim == 2;
+  for (int d = 0; d < 3; d++) G.pdx[d] = d < p->ndim ? p->dx[d] : 1.0;
 }
 
 // delx along every d depends only on the d coordinate: returns the spacing
@@ -653,6 +657,7 @@ extern "C" long long smaug_tape_doubles(const smaug_t* s) {
 extern "C" int smaug_step_taped(smaug_t* s, double* tape) {
   if (!s || !tape) return fail(SMAUG_E_ARG, "null argument");
   if (adjoint_gate("smaug_step_taped")) return SMAUG_E_UNSUPPORTED;
+  if (s->G.divb) return fail(SMAUG_E_UNSUPPORTED, "the adjoint does not include the divB source terms");
   return step_once(s, tape);
 }
 
@@ -663,6 +668,7 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
                                   double* gmap) {
   if (!s || !tape || !wnext || !lam) return fail(SMAUG_E_ARG, "null argument");
   if (adjoint_gate("smaug_adjoint_step")) return SMAUG_E_UNSUPPORTED;
+  if (s->G.divb) return fail(SMAUG_E_UNSUPPORTED, "the adjoint does not include the divB source terms");
   const long long N = s->N, F = (long long)s->np * N;
   if (!s->adj) {
     HIPCHK(hipMalloc(&s->adj, sizeof(double) * (size_t)(5 * F + N)));

@@ -739,6 +739,30 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
   double out[NP];
 #pragma unroll
   for (int f = 0; f < NP; f++) out[f] = at(A.base + (long long)f * N, c) + h * (double)acc[f];
+  // divB source terms (divb_parallel, src/divb.cu:165-338, 2D), in FP64 on
+  // cells 2 < i,j < n-2: divb from the 4th-order differences over 2 p->dx of
+  // b + b_b, zeroed on the layers 3, n-4, n-3 (grad3d's mask with interior
+  // transverse indices); m_d += h divb b_d, b_d += h divb m_d/rho_t (a NaN
+  // source counts 0); the energy source multiplies wd[bdotv], which the
+  // reference never fills, so it is 0
+  if (ND == 2 && G.divb && x[0] > 2 && x[1] > 2 && x[0] < G.n[0] - 2 && x[1] < G.n[1] - 2) {
+    auto g3 = [&](const double* a, int d) -> double {
+      const int xd = x[d], nn = G.n[d];
+      if (xd == 3 || xd == nn - 4 || xd == nn - 3) return 0.0;
+      const unsigned s = d == 0 ? 1u : (unsigned)G.n[0];
+      return ((8 * at(a, c + s) - 8 * at(a, c - s) + at(a, c - 2 * s) - at(a, c + 2 * s)) / 6.0) / (2.0 * G.pdx[d]);
+    };
+    const double divb = (g3(w + (long long)fB(ND, 0) * N, 0) + g3(w + (long long)fB(ND, 1) * N, 1)) +
+                        (g3(bg + (long long)bBB(0) * N, 0) + g3(bg + (long long)bBB(1) * N, 1));
+    const double rt = rho_c + rhob_c;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const double sm = -divb * at(w + (long long)fB(ND, d) * N, c);
+      const double sb = -divb * at(w + (long long)fMOM(d) * N, c) / rt;
+      out[fMOM(d)] -= h * (isnan(sm) ? 0.0 : sm);
+      out[fB(ND, d)] -= h * (isnan(sb) ? 0.0 : sb);
+    }
+  }
   if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_VERTICAL) {
     const double* dv = G.drv;
     double ez, exyz;

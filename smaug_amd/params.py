@@ -64,6 +64,7 @@ class SmaugParams(C.Structure):
         ("precision", C.c_int),
         ("qt0", C.c_double),
         ("it0", C.c_int),
+        ("divbfix", C.c_int),
     ]
 
 
@@ -102,6 +103,7 @@ class RunParams:
     moddton: int = 1                   # PMODDTON
     hyperdifmom: int = 1               # PHYPERDIFMOM
     divbon: int = 0                    # PDIVBON
+    divbfix: int = 0                   # PDIVBFIX
     chyp: Dict[str, float] = field(default_factory=dict)
     bcu: List[int] = field(default_factory=lambda: [0, 0, 0])   # BCU0..2
     bcl: List[int] = field(default_factory=lambda: [0, 0, 0])   # BCL0..2
@@ -163,6 +165,7 @@ class RunParams:
         p.precision = self.precision
         p.qt0 = self.qt0
         p.it0 = self.it0
+        p.divbfix = self.divbfix
         return p
 
 
@@ -200,7 +203,7 @@ def from_keys(kv: Dict[str, str]) -> RunParams:
         if key in kv:
             setattr(rp, attr, float(kv[key]))
     for key, attr in (("PRKON", "rkon"), ("PMODDTON", "moddton"), ("PHYPERDIFMOM", "hyperdifmom"),
-                      ("PDIVBON", "divbon"), ("DRIVER", "driver")):
+                      ("PDIVBON", "divbon"), ("PDIVBFIX", "divbfix"), ("DRIVER", "driver")):
         if key in kv:
             setattr(rp, attr, int(float(kv[key])))
     rp.g = [float(kv.get(f"PGRAV{d}", 0.0)) for d in range(3)]
@@ -234,8 +237,8 @@ def validate(rp: RunParams) -> Optional[str]:
         cells *= rp.ni[d] + 4
     if cells >= 1 << 29:
         return f"{cells} cells: grids of 2^29 cells or more are not supported"
-    if rp.divbon:
-        return "divbon=1 is not supported (every reference model sets PDIVBON 0)"
+    if rp.divbon not in (0, 1):
+        return "divbon must be 0 or 1"
     tables = [list(rp.bcu[: rp.ndim]), list(rp.bcl[: rp.ndim])]
     tables += [list(t[: rp.ndim]) for t in getattr(rp, "bc_field", {}).values()]
     for t in tables:

@@ -40,6 +40,14 @@ def make_case(name: str, terms: int = P.TERMS_ALL):
         amap = np.random.default_rng(3).uniform(150.0, 250.0, rp.ncells // rp.n[0])
         rp.terms = terms
         return rp, w, pos, amap, (7 if nd == 2 else 6), (3 if nd == 2 else 2)
+    if name == "s2d":  # 2D stratified RK4 with the divB source terms and a divergent b1
+        rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
+        rp.divbon, rp.divbfix = 1, 1
+        z = pos[0]
+        w = w.copy()
+        w[4] += 1e-3 * np.sin(2 * np.pi * (z - z.min()) / (z.max() - z.min())) * w[8]
+        rp.terms = terms
+        return rp, w, pos, None, 7, 3
     if name == "ot":
         rp, w, pos = ot_case(ni=16, rkon=1)
         rp.moddton = 0

@@ -57,9 +57,9 @@ def test_params_struct_layout_matches_c():
 #include <stdio.h>
 #include "smaug_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(smaug_params), offsetof(smaug_params, gamma),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(smaug_params), offsetof(smaug_params, gamma),
          offsetof(smaug_params, chyp), offsetof(smaug_params, boundtype), offsetof(smaug_params, drv),
-         offsetof(smaug_params, terms), offsetof(smaug_params, it0));
+         offsetof(smaug_params, terms), offsetof(smaug_params, it0), offsetof(smaug_params, divbfix));
   return 0;
 }
 '''
@@ -76,7 +76,7 @@ int main(void) {
                 os.remove(f)
     S = params.SmaugParams
     assert vals == [ctypes.sizeof(S), S.gamma.offset, S.chyp.offset, S.boundtype.offset, S.drv.offset,
-                    S.terms.offset, S.it0.offset]
+                    S.terms.offset, S.it0.offset, S.divbfix.offset]
 
 
 def test_param_text_uses_reference_names():
@@ -107,7 +107,6 @@ def test_param_text_rejects_malformed_lines():
 
 
 @pytest.mark.parametrize("mutate,needle", [
-    (lambda rp: setattr(rp, "divbon", 1), "divbon"),
     (lambda rp: setattr(rp, "bcu", [BC_PERIODIC] * 3), "periodic"),
     (lambda rp: setattr(rp, "bcu", [1, 4, 4]), "boundary type 1"),
     (lambda rp: setattr(rp, "bcu", [9, 4, 4]), "unknown boundary type 9"),
