@@ -126,6 +126,28 @@ int smaug_step(smaug_t* s, int nsteps);
  * of max(cfast_dir, c_sound)) and the time step the CFL rule would pick. */
 int smaug_cfl(smaug_t* s, double* cmax_per_dim, double* dt);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the computation of smaug_cfl on a caller's device state,
+ * with no context and no allocation of any kind.
+ *   w_device   -- NVAR*N doubles on the device, the layout of smaug_create's
+ *                 w0 (evolved fields, then the background), ghost cells
+ *                 filled;
+ *   out_device -- ndim + 2 doubles on the device: the wave-speed maxima per
+ *                 direction, the time step the CFL rule picks (p->dt when
+ *                 p->moddton is 0), and 1.0 when a wave speed is not finite
+ *                 (blow-up) else 0.0;
+ *   hip_stream -- the stream to queue on (NULL: the null stream).
+ * Work is only queued (no synchronisation): read out_device after the
+ * stream.  Its reduction runs in a fixed SMAUG_NOALLOC_SCRATCH_BYTES-byte
+ * scratch array that the library holds once per GPU until the process ends;
+ * the scratch does not grow with the grid, which is limited as smaug_create
+ * limits it.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (two streams, two threads) race and must be serialised by the
+ * caller. */
+#define SMAUG_NOALLOC_SCRATCH_BYTES 64
+int smaug_cfl_noalloc(const smaug_params* p, const double* w_device, double* out_device, void* hip_stream);
+
 /* Re-applies the boundary conditions to the current perturbation fields. */
 int smaug_fill_ghosts(smaug_t* s);
 

@@ -291,6 +291,12 @@ void launch_zero3(hipStream_t st, unsigned long long* p);
 void launch_clear_flag(hipStream_t st, int* flag);
 void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,
                double dtcap, int moddton, double* dts);
+// the scratch of smaug_cfl_noalloc, in 8-byte slots: the three maxima
+// (IEEE bits, atomicMax), the blow-up flag (an int in its slot), dt and t
+// of the CFL rule
+constexpr int kNaCmax = 0, kNaFlag = 3, kNaDts = 4, kNoallocSlots = 6;
+void launch_noalloc_init(hipStream_t st, unsigned long long* s, double dt0);
+void launch_noalloc_out(hipStream_t st, const unsigned long long* s, int nd, double* out);
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
 void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A);
 void launch_fill(const Geo& G, hipStream_t st, double* w);

@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -502,6 +503,56 @@ extern "C" int smaug_cfl(smaug_t* s, double* cmax_per_dim, double* dt) {
     *dt = v;
   }
   return check_flag(s);
+}
+
+// ---- smaug_cfl_noalloc: the CFL reduction with no context and no allocation.
+// Its scratch (maxima, flag, dt and t; slots kNa* of smaug_dev.h) is this
+// file-scope device array: one copy per GPU, held until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): it uses the address hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+static __device__ __align__(256) unsigned long long scratch[SMAUG_NOALLOC_SCRATCH_BYTES / 8];
+static_assert(kNoallocSlots * 8 <= SMAUG_NOALLOC_SCRATCH_BYTES, "scratch slots fit the stated size");
+
+static std::mutex g_scratch_mu;
+static constexpr int kMaxDevices = 64;
+static unsigned long long* g_scratch_ptr[kMaxDevices];
+
+static int scratch_ptr(unsigned long long** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SMAUG_E_UNSUPPORTED, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SMAUG_E_HIP, "hipGetSymbolAddress(scratch): %s (code %d)", hipGetErrorString(e), (int)e);
+    g_scratch_ptr[dev] = (unsigned long long*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_cfl_noalloc(const smaug_params* p, const double* w_device, double* out_device,
+                                 void* hip_stream) {
+  if (!p || !w_device || !out_device) return fail(SMAUG_E_ARG, "null argument");
+  int rc = check_params(p);
+  if (rc) return rc;
+  Geo G;
+  make_geo(p, G);
+  G.mixed = 0;  // wave speeds from the FP64 state; no shadow, no derived fields
+  unsigned long long* sc = nullptr;
+  rc = scratch_ptr(&sc);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)hip_stream;
+  const double* bg = w_device + (size_t)G.np * G.N;
+  launch_noalloc_init(st, sc, p->dt);  // src/iosmaug.c:549-550: every step starts from DT
+  launch_prim(G, st, w_device, bg, nullptr, nullptr, sc + kNaCmax, reinterpret_cast<int*>(sc + kNaFlag));
+  launch_dt(st, sc + kNaCmax, G.nd, p->dx, p->courant, p->dt, p->moddton, reinterpret_cast<double*>(sc + kNaDts));
+  launch_noalloc_out(st, sc, G.nd, out_device);
+  HIPCHK(hipGetLastError());
+  return SMAUG_OK;
 }
 
 extern "C" int smaug_fill_ghosts(smaug_t* s) {

@@ -70,9 +70,11 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
       bb2 = bb2 + bb[d] * bb[d];
       bbb_c = bbb_c + bb[d] * b[d];
     }
+    if (prim) {  // null: wave speeds only (smaug_cfl_noalloc)
 #pragma unroll
-    for (int d = 0; d < ND; d++) prim[(long long)pV(d) * N + c] = (SH)(m[d] / rt);
-    prim[(long long)pQE(ND) * N + c] = (SH)(e - 0.5 * (b2 + mm / rt));
+      for (int d = 0; d < ND; d++) prim[(long long)pV(d) * N + c] = (SH)(m[d] / rt);
+      prim[(long long)pQE(ND) * N + c] = (SH)(e - 0.5 * (b2 + mm / rt));
+    }
     if (shadow) {
       shadow[c] = (float)rho;
       shadow[(long long)fE(ND) * N + c] = (float)e;
@@ -225,6 +227,34 @@ void launch_zero3(hipStream_t st, unsigned long long* p) {
 }
 
 void launch_clear_flag(hipStream_t st, int* flag) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, flag); }
+
+// Scratch of the CFL reduction without a context (NoallocSlots layout):
+// maxima and flag to 0, (dt, t) to (dt0, 0).  A kernel, not a memset, so a
+// caller may capture it in a HIP graph.
+__global__ void k_noalloc_init(unsigned long long* s, double dt0) {
+  if (threadIdx.x < kNoallocSlots) {
+    unsigned long long v = 0ull;
+    if (threadIdx.x == kNaDts) v = (unsigned long long)__double_as_longlong(dt0);
+    s[threadIdx.x] = v;
+  }
+}
+
+// out[0..nd-1] = per-direction maxima, out[nd] = dt, out[nd+1] = 1 when a
+// wave speed was not finite, else 0
+__global__ void k_noalloc_out(const unsigned long long* s, int nd, double* out) {
+  if (threadIdx.x != 0) return;
+  for (int d = 0; d < nd; d++) out[d] = bits2d(s[kNaCmax + d]);
+  out[nd] = bits2d(s[kNaDts]);
+  out[nd + 1] = *reinterpret_cast<const int*>(s + kNaFlag) ? 1.0 : 0.0;
+}
+
+void launch_noalloc_init(hipStream_t st, unsigned long long* s, double dt0) {
+  hipLaunchKernelGGL(k_noalloc_init, dim3(1), dim3(64), 0, st, s, dt0);
+}
+
+void launch_noalloc_out(hipStream_t st, const unsigned long long* s, int nd, double* out) {
+  hipLaunchKernelGGL(k_noalloc_out, dim3(1), dim3(64), 0, st, s, nd, out);
+}
 
 void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const double* dx, double courant,
                double dtcap, int moddton, double* dts) {

@@ -24,7 +24,7 @@ EXPORTS = [
     "smaug_sync", "smaug_destroy", "smaug_last_error", "smaug_version", "smaug_time_kernels",
     "smaug_ini_read", "smaug_ini_write", "smaug_out_write", "smaug_write_out",
     "smaug_set_driver_map", "smaug_set_stream", "smaug_set_clock", "smaug_save_pert", "smaug_load_pert",
-    "smaug_tape_doubles", "smaug_step_taped", "smaug_adjoint_step",
+    "smaug_tape_doubles", "smaug_step_taped", "smaug_adjoint_step", "smaug_cfl_noalloc",
 ]
 
 
@@ -69,6 +69,7 @@ def load() -> ctypes.CDLL:
     L.smaug_tape_doubles.argtypes = [vp]
     L.smaug_step_taped.argtypes = [vp, vp]
     L.smaug_adjoint_step.argtypes = [vp, vp, vp, vp, vp]
+    L.smaug_cfl_noalloc.argtypes = [vp, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     L.smaug_last_error.restype = ctypes.c_char_p
@@ -82,6 +83,17 @@ def _chk(rc: int, what: str):
     if rc != 0:
         msg = load().smaug_last_error().decode(errors="replace")
         raise SmaugError(f"{what} failed ({rc}): {msg}")
+
+
+def cfl_noalloc(rp: RunParams, w_ptr: int, out_ptr: int, stream_ptr: int = 0):
+    """smaug_cfl_noalloc: the wave-speed maxima per direction, the CFL step
+    and the blow-up flag of a device state (NVAR*N doubles at w_ptr, the
+    layout of Engine's w0) into ndim + 2 doubles at out_ptr, queued on the
+    HIP stream stream_ptr (0: the null stream) without allocating anything.
+    Read out_ptr after synchronising that stream."""
+    cp = rp.to_c()
+    _chk(load().smaug_cfl_noalloc(ctypes.byref(cp), ctypes.c_void_p(w_ptr), ctypes.c_void_p(out_ptr),
+                                  ctypes.c_void_p(stream_ptr or None)), "smaug_cfl_noalloc")
 
 
 def _dp(a: np.ndarray):

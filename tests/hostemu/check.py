@@ -276,9 +276,36 @@ def check_blowup(kind: str):
     return out
 
 
+def check_noalloc():
+    """smaug_cfl_noalloc against smaug_cfl on two states in a row, the
+    second on a smaller grid, the states in guarded caller buffers."""
+    from smaug_amd.engine import cfl_noalloc
+
+    out = []
+    for ni, nd, moddton in (((14, 10, 8), 3, 1), ((12, 8), 2, 0)):
+        rp, w, pos = strat_case(ni=ni, ndim=nd, rkon=1)
+        rp.moddton = moddton
+        rp.courant = 0.3
+        if moddton:
+            rp.dt = 10.0  # above the CFL step, so the rule picks courant / courantmax
+        e = Engine(rp, w, pos)
+        cm, dt = e.cfl()
+        state = e.state()
+        e.close()
+        wd = caller_buffer(state.size)
+        wd[:] = state.ravel()
+        res = caller_buffer(nd + 2, np.nan)
+        cfl_noalloc(rp, wd.ctypes.data, res.ctypes.data)
+        out.append({"ndim": nd, "cmax": list(map(float, cm)), "dt": float(dt),
+                    "got": list(map(float, res))})
+    return out
+
+
 def main(argv):
     what = argv[0]
-    if what == "blowup":
+    if what == "noalloc":
+        r = check_noalloc()
+    elif what == "blowup":
         r = check_blowup(argv[1])
     elif what == "fwd":
         r = check_fwd(argv[1])

@@ -25,6 +25,7 @@
 #define __host__
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
 
 typedef int hipError_t;
 typedef void* hipStream_t;
@@ -115,6 +116,16 @@ inline hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFl
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+// a file-scope "device" array is host memory here, its address its name's
+#define HIP_SYMBOL(s) (&(s))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 inline const char* hipGetErrorString(hipError_t) { return "emu"; }
 inline hipError_t hipEventCreate(hipEvent_t* e) {
   *e = (void*)0x1;
