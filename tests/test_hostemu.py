@@ -53,7 +53,7 @@ def close(pair):
     return abs(got - want) <= TOL * abs(want)
 
 
-@pytest.mark.parametrize("case", ["s2r", "s2e", "s3r", "ot", "s2d"])
+@pytest.mark.parametrize("case", ["s2r", "s2e", "s3r", "s3m", "ot", "s2d"])
 @pytest.mark.parametrize("align", ["end", "start"])
 def test_forward_matches_oracle(hostlib, case, align):
     r = run(hostlib, "fwd", case, align=align)
