@@ -216,7 +216,11 @@ int smaug_write_out(smaug_t* s, const char* prefix, const char* name);
  *   smaug_save_pert / smaug_load_pert -- device-to-device copy of the current
  *                            evolved fields (np*N doubles, np = 2+2*ndim) to /
  *                            from a caller's device buffer; load re-fills the
- *                            ghost cells. */
+ *                            ghost cells.
+ * Every entry point that takes a caller's device buffer (these, the adjoint
+ * set below and smaug_cfl_noalloc) first checks that the buffer lies inside
+ * one device allocation large enough for the call and returns SMAUG_E_ARG
+ * otherwise. */
 int smaug_set_driver_map(smaug_t* s, const double* amap, int on_device);
 int smaug_set_stream(smaug_t* s, void* hip_stream);
 int smaug_set_clock(smaug_t* s, int it, double qt);

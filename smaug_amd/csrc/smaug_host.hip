@@ -50,6 +50,24 @@ int io_fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(SMAUG_E_HIP, "%s (code %d)", hipGetErrorString(e_), (int)e_); \
   } while (0)
 
+// A caller's device buffer of `bytes` bytes at p must lie inside one device
+// allocation (a pointer torch or another allocator hands out, or one of this
+// library's); otherwise the call is refused before anything is queued.
+static int check_range(const void* p, size_t bytes, const char* what) {
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  const hipError_t e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // the failed query leaves no error for the next call
+    return fail(SMAUG_E_ARG, "%s (%p) is not device memory: %s", what, p, hipGetErrorString(e));
+  }
+  const char* end = (const char*)base + size;
+  if ((const char*)p + bytes > end)
+    return fail(SMAUG_E_ARG, "%s: %zu bytes at %p run %zu bytes past the end of its allocation", what, bytes, p,
+                (size_t)((const char*)p + bytes - end));
+  return SMAUG_OK;
+}
+
 struct smaug_ctx {
   smaug_params p;
   Geo G;
@@ -545,6 +563,9 @@ extern "C" int smaug_cfl_noalloc(const smaug_params* p, const double* w_device, 
   unsigned long long* sc = nullptr;
   rc = scratch_ptr(&sc);
   if (rc) return rc;
+  rc = check_range(w_device, sizeof(double) * (size_t)(G.np + 2 + G.nd) * G.N, "w_device");
+  if (!rc) rc = check_range(out_device, sizeof(double) * (G.nd + 2), "out_device");
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)hip_stream;
   const double* bg = w_device + (size_t)G.np * G.N;
   launch_noalloc_init(st, sc, p->dt);  // src/iosmaug.c:549-550: every step starts from DT
@@ -655,6 +676,8 @@ extern "C" int smaug_set_clock(smaug_t* s, int it, double qt) {
 extern "C" int smaug_save_pert(const smaug_t* cs, double* dst) {
   smaug_t* s = const_cast<smaug_t*>(cs);
   if (!s || !dst) return fail(SMAUG_E_ARG, "null argument");
+  int rc = check_range(dst, sizeof(double) * (size_t)s->np * s->N, "smaug_save_pert destination");
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(dst, s->U[s->cur], sizeof(double) * (size_t)s->np * s->N, hipMemcpyDeviceToDevice,
                         s->stream));
   return SMAUG_OK;
@@ -662,6 +685,8 @@ extern "C" int smaug_save_pert(const smaug_t* cs, double* dst) {
 
 extern "C" int smaug_load_pert(smaug_t* s, const double* src) {
   if (!s || !src) return fail(SMAUG_E_ARG, "null argument");
+  int rc = check_range(src, sizeof(double) * (size_t)s->np * s->N, "smaug_load_pert source");
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(s->U[s->cur], src, sizeof(double) * (size_t)s->np * s->N, hipMemcpyDeviceToDevice,
                         s->stream));
   launch_fill(s->G, s->stream, s->U[s->cur]);
@@ -693,6 +718,10 @@ extern "C" int smaug_set_driver_map(smaug_t* s, const double* amap, int on_devic
   }
   if (s->p.driver != SMAUG_DRIVER_VERTICAL)
     return fail(SMAUG_E_ARG, "a driver amplitude map needs driver = SMAUG_DRIVER_VERTICAL");
+  if (on_device) {
+    const int rc = check_range(amap, bytes, "driver amplitude map");
+    if (rc) return rc;
+  }
   if (!s->amap) HIPCHK(hipMalloc(&s->amap, bytes));
   HIPCHK(hipMemcpyAsync(s->amap, amap, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         s->stream));
@@ -709,6 +738,8 @@ extern "C" int smaug_step_taped(smaug_t* s, double* tape) {
   if (!s || !tape) return fail(SMAUG_E_ARG, "null argument");
   if (adjoint_gate("smaug_step_taped")) return SMAUG_E_UNSUPPORTED;
   if (s->G.divb) return fail(SMAUG_E_UNSUPPORTED, "the adjoint does not include the divB source terms");
+  const int rc = check_range(tape, sizeof(double) * (size_t)smaug_tape_doubles(s), "tape");
+  if (rc) return rc;
   return step_once(s, tape);
 }
 
@@ -721,6 +752,14 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
   if (adjoint_gate("smaug_adjoint_step")) return SMAUG_E_UNSUPPORTED;
   if (s->G.divb) return fail(SMAUG_E_UNSUPPORTED, "the adjoint does not include the divB source terms");
   const long long N = s->N, F = (long long)s->np * N;
+  {
+    const size_t fb = sizeof(double) * (size_t)F;
+    int rc = check_range(tape, sizeof(double) * (size_t)smaug_tape_doubles(s), "tape");
+    if (!rc) rc = check_range(wnext, fb, "wnext");
+    if (!rc) rc = check_range(lam, fb, "lam");
+    if (!rc && gmap) rc = check_range(gmap, sizeof(double) * (size_t)(N / s->G.n[0]), "gmap");
+    if (rc) return rc;
+  }
   if (!s->adj) {
     HIPCHK(hipMalloc(&s->adj, sizeof(double) * (size_t)(5 * F + N)));
     HIPCHK(hipMalloc(&s->aprim, sizeof(double) * (size_t)(s->npr * N)));
@@ -737,8 +776,13 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
   unsigned long long* cmx = s->cmax + 12;
   const int nst = s->p.rkon == 1 ? 4 : 1;
   const double* meta = tape + nst * F;
+  // every launch is checked as it is queued, so a failure names its kernel
+  auto launched = [&](const char* what) -> int {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SMAUG_OK : fail(SMAUG_E_HIP, "%s launch: %s (code %d)", what, hipGetErrorString(e), (int)e);
+  };
   auto stage = [&](const double* win, const double* lam_out, const double* outr, bool comb, double hfrac,
-                   double cb, int base_set, double* target, bool first) {
+                   double cb, int base_set, double* target, bool first) -> int {
     AdjMuArgs M;
     memset(&M, 0, sizeof M);
     M.lam_out = lam_out;
@@ -764,8 +808,12 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
     M.gcell = gcell;
     M.gcell_set = first ? 1 : 0;
     launch_adj_mu(s->G, st, M);
+    int rc = launched("k_adj_mu");
+    if (rc) return rc;
     launch_zero3(st, cmx);
+    if ((rc = launched("k_zero3"))) return rc;
     launch_prim(Gd, st, win, s->bg, s->aprim, nullptr, cmx, s->flag);
+    if ((rc = launched("k_prim (adjoint stage input)"))) return rc;
     AdjArgs R;
     R.win = win;
     R.bg = s->bg;
@@ -778,20 +826,26 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
     R.cmax_bits = cmx;
     R.out = T;
     launch_adj_rhs(s->G, st, R);
+    if ((rc = launched("k_adj_rhs"))) return rc;
     launch_adj_fold(s->G, st, T, target);
+    return launched("k_adj_fold");
   };
+  int rc = SMAUG_OK;
   if (s->p.rkon == 0) {
-    stage(tape, lam, wnext, false, 1.0, 0.0, 1, lam, true);
+    rc = stage(tape, lam, wnext, false, 1.0, 0.0, 1, lam, true);
   } else {
     const double *w0 = tape, *s1 = tape + F, *s2 = tape + 2 * F, *s3 = tape + 3 * F;
     // final stage: out = w0 + h/6 acc(s3) + src, w^{n+1} = out + (s1 + 2 s2 + s3 - 4 w0)/3
-    stage(s3, lam, wnext, true, 1.0 / 6.0, -4.0 / 3.0, 1, L3, true);
-    stage(s2, L3, s3, false, 1.0, 0.0, 0, L2, false);
-    stage(s1, L2, s2, false, 0.5, 0.0, 0, L1, false);
-    stage(w0, L1, s1, false, 0.5, 0.0, 0, lam, false);
+    rc = stage(s3, lam, wnext, true, 1.0 / 6.0, -4.0 / 3.0, 1, L3, true);
+    if (!rc) rc = stage(s2, L3, s3, false, 1.0, 0.0, 0, L2, false);
+    if (!rc) rc = stage(s1, L2, s2, false, 0.5, 0.0, 0, L1, false);
+    if (!rc) rc = stage(w0, L1, s1, false, 0.5, 0.0, 0, lam, false);
   }
-  if (gmap) launch_colsum(s->G, st, gcell, gmap);
-  HIPCHK(hipGetLastError());
+  if (rc) return rc;
+  if (gmap) {
+    launch_colsum(s->G, st, gcell, gmap);
+    return launched("k_colsum");
+  }
   return SMAUG_OK;
 }
 

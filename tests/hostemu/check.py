@@ -305,9 +305,51 @@ def check_noalloc():
     return out
 
 
+def check_ranges():
+    """Caller device buffers shorter than the call needs are refused with
+    SMAUG_E_ARG before anything is queued (the short buffers are guarded
+    host-build allocations, so a kernel or copy that ran anyway would stop
+    the check at the guard page)."""
+    from smaug_amd.engine import SmaugError, cfl_noalloc
+
+    rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
+    k = P.npert(rp.ndim)
+    F = k * rp.ncells
+    e = Engine(rp, w, pos)
+    out = {}
+
+    def refused(name, fn):
+        try:
+            fn()
+            out[name] = "accepted"
+        except SmaugError as err:
+            out[name] = str(err)
+
+    short = caller_buffer(F - 1)
+    refused("save_pert", lambda: e.save_pert(short.ctypes.data))
+    refused("load_pert", lambda: e.load_pert(short.ctypes.data))
+    tape = caller_buffer(e.tape_doubles() - 1)
+    refused("step_taped", lambda: e.step_taped(tape.ctypes.data))
+    good_tape = caller_buffer(e.tape_doubles())
+    e.step_taped(good_tape.ctypes.data)
+    wn = caller_buffer(F)
+    e.save_pert(wn.ctypes.data)
+    refused("adjoint_step", lambda: e.adjoint_step(good_tape.ctypes.data, wn.ctypes.data, short.ctypes.data))
+    amap = caller_buffer(rp.ncells // rp.n[0] - 1)
+    refused("set_driver_map", lambda: e.set_driver_map(device_ptr=amap.ctypes.data))
+    state = caller_buffer(P.nvar(rp.ndim) * rp.ncells)
+    state[:] = e.state().ravel()
+    res = caller_buffer(rp.ndim + 1)
+    refused("cfl_noalloc", lambda: cfl_noalloc(rp, state.ctypes.data, res.ctypes.data))
+    e.close()
+    return out
+
+
 def main(argv):
     what = argv[0]
-    if what == "noalloc":
+    if what == "ranges":
+        r = check_ranges()
+    elif what == "noalloc":
         r = check_noalloc()
     elif what == "blowup":
         r = check_blowup(argv[1])

@@ -57,6 +57,7 @@ void launch(dim3 grid, dim3 block, std::function<void()> body) {
 struct Map {
   char* base;
   size_t len;
+  size_t n;  // bytes asked for
 };
 static std::map<void*, Map> g_maps;
 hipError_t hipMalloc(void** p, size_t n) {
@@ -72,7 +73,7 @@ hipError_t hipMalloc(void** p, size_t n) {
   const char* al = getenv("EMU_ALIGN");
   char* q = (al && !strcmp(al, "start")) ? base + pg : base + pg + data - ((n + 7) / 8 * 8);
   memset(q, 0xff, n);  // NaN garbage like fresh device memory might hold
-  g_maps[q] = Map{base, len};
+  g_maps[q] = Map{base, len, n};
   *p = q;
   return hipSuccess;
 }
@@ -82,5 +83,21 @@ hipError_t hipFree(void* p) {
   if (it == g_maps.end()) return 1;
   munmap(it->second.base, it->second.len);
   g_maps.erase(it);
+  return hipSuccess;
+}
+hipError_t hipMemGetAddressRange(hipDeviceptr_t* base, size_t* size, hipDeviceptr_t p) {
+  const char* c = (const char*)p;
+  auto it = g_maps.upper_bound(p);
+  if (it != g_maps.begin()) {
+    --it;
+    const char* q = (const char*)it->first;
+    if (c >= q && c < q + it->second.n) {
+      *base = it->first;
+      *size = it->second.n;
+      return hipSuccess;
+    }
+  }
+  *base = p;
+  *size = (size_t)1 << 62;
   return hipSuccess;
 }

@@ -101,6 +101,11 @@ hipError_t hipMalloc(T** p, size_t n) {
   return hipMalloc((void**)p, n);
 }
 hipError_t hipFree(void* p);
+typedef void* hipDeviceptr_t;
+// the allocation of this emulation's hipMalloc holding p; memory it did not
+// allocate (host arrays the checks pass as device buffers) is reported as a
+// range it cannot bound
+hipError_t hipMemGetAddressRange(hipDeviceptr_t* base, size_t* size, hipDeviceptr_t p);
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
   memmove(d, s, n);
   return hipSuccess;

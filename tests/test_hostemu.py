@@ -126,3 +126,13 @@ def test_autograd_function_and_checkpointing(hostlib, rkon):
     assert r["every_identical"]
     assert close(r["w0"]), r["w0"]
     assert close(r["amap"]), r["amap"]
+
+
+def test_short_caller_buffers_are_refused(hostlib):
+    """Every entry point that takes a caller's device buffer checks that the
+    buffer holds what the call reads or writes (hipMemGetAddressRange) and
+    returns SMAUG_E_ARG otherwise."""
+    r = run(hostlib, "ranges")
+    assert set(r) == {"save_pert", "load_pert", "step_taped", "adjoint_step", "set_driver_map", "cfl_noalloc"}
+    for name, msg in r.items():
+        assert "failed (-1)" in msg and "past the end of its allocation" in msg, (name, msg)
