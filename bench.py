@@ -95,9 +95,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--mixed-check-steps", type=int, default=0,
-                    help="also run FP64 and mixed from the initial state for this many steps (2000 for the "
-                         "mixed-precision configuration) and report their per-variable differences")
+    ap.add_argument("--mixed-check-steps", type=int, default=2000,
+                    help="also run FP64, mixed and FP64 from a one-ulp-perturbed state from the initial state "
+                         "for this many steps (2000: the horizon of the mixed-precision configuration) and "
+                         "report the per-variable differences; 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 256 MiB -> 256 MiB elementwise copy first (HBM counter calibration)")
     args = ap.parse_args(argv)
