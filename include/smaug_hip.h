@@ -70,9 +70,14 @@ extern "C" {
 #define SMAUG_T_SOURCE 0x80    /* user source (driver) plug-in             */
 #define SMAUG_TERMS_ALL 0xFF
 
-/* user-source plug-ins (reference models/usersource.cu.*) */
+/* user-source plug-ins (reference models/usersource.cu.*), the
+ * addsourceterms2 of a model applied to the post-update value with the
+ * stage's dt (src/centdiff2.cu:790-872); parameters in drv[] below */
 #define SMAUG_DRIVER_NONE 0       /* usersource.cu.default                 */
 #define SMAUG_DRIVER_VERTICAL 1   /* Gaussian vertical-velocity driver     */
+#define SMAUG_DRIVER_TORSIONAL 2  /* usersource.cu.fluxtube1 (3D)          */
+#define SMAUG_DRIVER_SPICULE1 3   /* usersource.cu.spicule1                */
+#define SMAUG_NDRV 12
 
 /* arithmetic of the evolved state */
 #define SMAUG_PREC_FP64 0
@@ -96,11 +101,23 @@ typedef struct smaug_params {
   double dx[3];             /* p->dx = (xmax-xmin)/NI, used by the CFL rule  */
   int boundtype[SMAUG_MAXVAR][3][2]; /* [field][dir][0=upper BCU,1=lower BCL] */
   int driver;               /* SMAUG_DRIVER_*                                */
-  double drv[8];            /* SMAUG_DRIVER_VERTICAL: A [m/s], period P [s], z0,
-                               x0, y0 [m], sigma_z, sigma_xy [m], unused;
-                               v_z = A exp(-dz^2/sz^2 - dx^2/sxy^2 [- dy^2/sxy^2])
-                               sin(2 pi qt / P), m_0 += dt v_z rho_t,
-                               e += dt v_z^2 rho_t / 2                       */
+  double drv[SMAUG_NDRV];   /* driver parameters (positions are the cells' wd
+                               positions unless stated; dims 0, 1, 2):
+     VERTICAL: A [m/s], period P [s], z0, x0, y0 [m], sigma_z, sigma_xy [m];
+       v_z = A exp(-dz^2/sz^2 - dx^2/sxy^2 [- dy^2/sxy^2]) sin(2 pi qt / P),
+       m_0 += dt v_z rho_t, e += dt v_z^2 rho_t / 2
+     TORSIONAL (models/usersource.cu.fluxtube1:3-90): aa [m/s], tdep, xc1,
+       xc2, xc3 [m], dz, dx, dy [m] (Gaussian widths along dims 0, 1, 2),
+       xxmax, yymax [m]; with zp, xp, yp the offsets from xc1, xc2, xc3 along
+       dims 0, 1, 2 and E = exp(-xp^2/dx^2) exp(-yp^2/dy^2) exp(-zp^2/dz^2):
+       v_1 = (aa yp / yymax) E tdep, v_2 = -(aa xp / xxmax) E tdep,
+       m_1 += dt v_1 rho_t, m_2 += dt v_2 rho_t,
+       e += dt (v_1^2 + v_2^2) rho_t / 2                          (3D only)
+     SPICULE1 (models/usersource.cu.spicule1:4-136): AA [m/s], tdep, xc1,
+       xc2 [m], unused, delta_z, delta_x [m]; only in the step whose counter
+       p->it is 1, at the grid positions xmin[d] + i_d dx[d] (i_d counting
+       ghost cells): v = AA exp(-(x0-xc1)^2/delta_z^2)
+       exp(-(x1-xc2)^2/delta_x^2) tdep, m_0 += dt v rho_t             */
   unsigned terms;           /* SMAUG_T_* mask, SMAUG_TERMS_ALL normally      */
   int precision;            /* SMAUG_PREC_*                                  */
   double qt0;               /* initial value of the nominal clock p->qt      */

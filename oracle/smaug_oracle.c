@@ -1027,7 +1027,65 @@ static void k_hbne(orc_t* o, int so, int sn, int field, int dim, int jj, int ii0
 /* source_parallel -> addsourceterms2 (src/centdiff2.cu:790-872) with the
  * vertical-velocity driver; the form follows models/usersource.cu.fluxtube1:
  * m += dt*v*rho_t, e += dt*v^2*rho_t/2, positions from wd, time from qt. */
+/* models/usersource.cu.fluxtube1:3-90 (3D): torsional velocity about the
+ * dim-0 axis through (xc2, xc3), positions from wd, constant tdep;
+ * m_1, m_2 and e updated in the reference's order and expression forms */
+static void k_source_torsional(orc_t* o, int sn) {
+  const double* dv = o->p.drv;
+  const double aa = dv[0], tdep = dv[1], xc1 = dv[2], xc2 = dv[3], xc3 = dv[4];
+  const double dz = dv[5], dx = dv[6], dy = dv[7], xxmax = dv[8], yymax = dv[9];
+  const double dt = o->dt;
+#pragma omp parallel for collapse(2)
+  FOR_ALL(o) {
+    long c = enc(o, i, j, k);
+    double xp = WD(o, D_POS(1))[c] - xc2;
+    double zp = WD(o, D_POS(0))[c] - xc1;
+    double yp = WD(o, D_POS(2))[c] - xc3;
+    double exp_z = exp(-zp * zp / (dz * dz));
+    double exp_x = exp(-xp * xp / (dx * dx));
+    double exp_y = exp(-yp * yp / (dy * dy));
+    double exp_xyz = exp_x * exp_y * exp_z;
+    double vx = (aa * yp / yymax) * exp_xyz * tdep;
+    double vy = -(aa * xp / xxmax) * exp_xyz * tdep;
+    W(o, sn, F_MOM(1))[c] += (dt)*vx * (W(o, sn, F_RHO)[c] + W(o, sn, F_RHOB)[c]);
+    W(o, sn, F_MOM(2))[c] += (dt)*vy * (W(o, sn, F_RHO)[c] + W(o, sn, F_RHOB)[c]);
+    W(o, sn, F_E)[c] += (dt) * (vx * vx + vy * vy) * (W(o, sn, F_RHO)[c] + W(o, sn, F_RHOB)[c]) / 2.0;
+  }
+}
+
+/* models/usersource.cu.spicule1:4-136: only in the step whose counter
+ * p->it is 1 (:67), positions from the grid origin and p->dx with the
+ * index counting ghost cells (:69-70), m_0 only (the energy term is
+ * commented out, :133) */
+static void k_source_spicule1(orc_t* o, int sn) {
+  if (o->it != 1) return;
+  const double* dv = o->p.drv;
+  const double AA = dv[0], tdep = dv[1], xc1 = dv[2], xc2 = dv[3], delta_z = dv[5], delta_x = dv[6];
+  const double dt = o->dt;
+#pragma omp parallel for collapse(2)
+  FOR_ALL(o) {
+    long c = enc(o, i, j, k);
+    double xp = (o->p.xmin[0]) + (((double)i) * (o->p.dx[0]));
+    double yp = (o->p.xmin[1]) + (((double)j) * (o->p.dx[1]));
+    double r1 = (xp - xc1) * (xp - xc1);
+    double r2 = (yp - xc2) * (yp - xc2);
+    double exp_z = exp(-r1 / (delta_z * delta_z));
+    double exp_x = exp(-r2 / (delta_x * delta_x));
+    double exp_xyz = exp_x * exp_z;
+    double vvz = AA * exp_xyz * tdep;
+    W(o, sn, F_MOM(0))[c] = W(o, sn, F_MOM(0))[c] + (dt)*vvz * (W(o, sn, F_RHO)[c] + W(o, sn, F_RHOB)[c]);
+  }
+}
+
 static void k_source(orc_t* o, int sn) {
+  if (o->p.driver == SMAUG_DRIVER_TORSIONAL) {
+    k_source_torsional(o, sn);
+    return;
+  }
+  if (o->p.driver == SMAUG_DRIVER_SPICULE1) {
+    k_source_spicule1(o, sn);
+    return;
+  }
   if (o->p.driver != SMAUG_DRIVER_VERTICAL) return;
   const double* dv = o->p.drv;
   const double A = dv[0], P = dv[1], z0 = dv[2], x0 = dv[3], y0 = dv[4], sz = dv[5], sxy = dv[6];

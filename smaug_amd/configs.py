@@ -21,7 +21,8 @@ from typing import Optional, Tuple
 
 import numpy as np
 
-from .params import (BC_CONTCD4, BC_PERIODIC, DRIVER_VERTICAL, NGHOST, RunParams, nvar)
+from .params import (BC_CONTCD4, BC_PERIODIC, DRIVER_SPICULE1, DRIVER_TORSIONAL, DRIVER_VERTICAL, NDRV, NGHOST,
+                     RunParams, nvar)
 
 MU0 = 4.0e-7 * math.pi
 R_GAS = 8.31e3
@@ -130,13 +131,35 @@ def stratified_params(ni=(128, 128, 128), ndim: int = 3, rkon: int = 1, courant:
     ni = list(ni)[:ndim]
     xmin = [0.1e6, 0.0, 0.0][:ndim]
     xmax = [6.0e6, 4.0e6, 4.0e6][:ndim]
-    drv = [amplitude, 300.0, 0.1e6, 2.0e6, 2.0e6, 0.05e6, 0.1e6, 0.0]
+    drv = [amplitude, 300.0, 0.1e6, 2.0e6, 2.0e6, 0.05e6, 0.1e6] + [0.0] * (NDRV - 7)
     return RunParams(
         ndim=ndim, ni=ni, xmin=xmin, xmax=xmax, dt=stratified_dt(ni, ndim, courant, b_tesla),
         gamma=5.0 / 3.0, g=[-274.0, 0.0, 0.0], courant=courant, rkon=rkon, moddton=0, hyperdifmom=1,
         chyp={"default": 0.02, "rho": 0.02, "energy": 0.02, "b1": 0.02, "b2": 0.02, "b3": 0.02,
               "mom1": 0.2, "mom2": 0.2, "mom3": 0.2},
         bcu=[BC_CONTCD4] * 3, bcl=[BC_CONTCD4] * 3, driver=DRIVER_VERTICAL, drv=drv)
+
+
+def fluxtube1_driver(rp: RunParams, aa: float = 10.0, tdep: float = 1.0, centre=(0.1e6, 1.0e6, 1.0e6),
+                     widths=(0.05e6, 0.1e6, 0.1e6), scales=(2.0e6, 2.0e6)) -> RunParams:
+    """The torsional driver of the fluxtube1 model on rp (3D): the constants of
+    models/usersource.cu.fluxtube1:28-54 by default (aa 10 m/s, tdep 1,
+    centre (0.1, 1, 1) Mm along dims 0, 1, 2, Gaussian widths 0.05, 0.1,
+    0.1 Mm, xxmax = yymax = 2 Mm)."""
+    rp.driver = DRIVER_TORSIONAL
+    rp.drv = [aa, tdep, *centre, *widths, *scales] + [0.0] * (NDRV - 10)
+    return rp
+
+
+def spicule1_driver(rp: RunParams, AA: float = 200.0, tdep: float = 1.0, centre=(0.0174e6, 1.998046e6),
+                    widths=(0.001e6, 0.001e6)) -> RunParams:
+    """The spicule1 model's driver on rp: the constants of
+    models/usersource.cu.spicule1:32-65 by default (AA 200 m/s, tdep 1,
+    centre (0.0174, 1.998046) Mm along dims 0, 1, widths delta_z =
+    delta_x = 0.001 Mm); it acts in the step whose counter is 1 only."""
+    rp.driver = DRIVER_SPICULE1
+    rp.drv = [AA, tdep, centre[0], centre[1], 0.0, widths[0], widths[1]] + [0.0] * (NDRV - 7)
+    return rp
 
 
 def stratified(rp: RunParams, b_tesla: float = 0.01, perturb: Optional[int] = None,

@@ -222,6 +222,23 @@ __global__ __launch_bounds__(256) void k_adj_mu(Geo G, AdjMuArgs A) {
     lp[fRHO()] = lo[fRHO()] + h * vz * lo[fMOM(0)] + h * vz * vz * 0.5 * lo[fE(ND)];
     gc = (lo[fMOM(0)] * h * rto + lo[fE(ND)] * h * vz * rto) * shape;
   }
+  // torsional driver: m_1 += h v_1 rho_t, m_2 += h v_2 rho_t,
+  // e += h (v_1^2 + v_2^2) rho_t / 2, rho_t of the updated value
+  if (ND == 3 && in && (G.terms & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_TORSIONAL) {
+    const double* dv = G.drv;
+    const double xp = A.pos[N + c] - dv[3], zp = A.pos[c] - dv[2], yp = A.pos[2 * N + c] - dv[4];
+    const double exyz = exp(-xp * xp / (dv[6] * dv[6])) * exp(-yp * yp / (dv[7] * dv[7])) * exp(-zp * zp / (dv[5] * dv[5]));
+    const double vx = (dv[0] * yp / dv[9]) * exyz * tdep, vy = -(dv[0] * xp / dv[8]) * exyz * tdep;
+    lp[fRHO()] = lo[fRHO()] + h * vx * lo[fMOM(1)] + h * vy * lo[fMOM(ND - 1)] + h * (vx * vx + vy * vy) * 0.5 * lo[fE(ND)];
+  }
+  // spicule1: m_0 += h v rho_t in the step whose counter is 1 (tdep != 0)
+  if (in && (G.terms & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_SPICULE1 && tdep != 0.0) {
+    const double* dv = G.drv;
+    const double xp = G.pxmin[0] + (double)x[0] * G.pdx[0], yp = G.pxmin[1] + (double)x[1] * G.pdx[1];
+    const double r1 = (xp - dv[2]) * (xp - dv[2]), r2 = (yp - dv[3]) * (yp - dv[3]);
+    const double vvz = dv[0] * (exp(-r2 / (dv[6] * dv[6])) * exp(-r1 / (dv[5] * dv[5]))) * tdep;
+    lp[fRHO()] = lo[fRHO()] + h * vvz * lo[fMOM(0)];
+  }
 #pragma unroll
   for (int f = 0; f < NP; f++) {
     const long long q = (long long)f * N + c;

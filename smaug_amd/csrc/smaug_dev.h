@@ -33,10 +33,11 @@ struct Geo {
   unsigned terms;        // SMAUG_T_* mask
   int hyper;             // PHYPERDIFMOM
   int driver;
-  double drv[8];
+  double drv[SMAUG_NDRV];
   int mixed;             // 1: stencils read FP32 shadows (SMAUG_PREC_MIXED)
   int divb;              // 1: divB source terms (divbon=1, divbfix != 0, 2D)
   double pdx[3];         // p->dx = (xmax-xmin)/NI (the divB gradients' spacing)
+  double pxmin[3];       // p->xmin (with pdx, the spicule1 driver's grid positions)
 };
 
 // evolved-field indices

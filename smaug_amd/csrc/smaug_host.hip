@@ -175,10 +175,16 @@ static int check_params(const smaug_params* p) {
   if (p->rkon != 0 && p->rkon != 1) return fail(SMAUG_E_ARG, "rkon must be 0 or 1");
   if (p->precision != SMAUG_PREC_FP64 && p->precision != SMAUG_PREC_MIXED)
     return fail(SMAUG_E_ARG, "unknown precision");
-  if (p->driver != SMAUG_DRIVER_NONE && p->driver != SMAUG_DRIVER_VERTICAL)
-    return fail(SMAUG_E_ARG, "unknown driver");
+  if (p->driver < SMAUG_DRIVER_NONE || p->driver > SMAUG_DRIVER_SPICULE1) return fail(SMAUG_E_ARG, "unknown driver");
   if (p->driver == SMAUG_DRIVER_VERTICAL && (p->drv[1] == 0 || p->drv[5] == 0 || p->drv[6] == 0))
     return fail(SMAUG_E_ARG, "driver needs a period and widths (drv[1], drv[5], drv[6])");
+  if (p->driver == SMAUG_DRIVER_TORSIONAL) {
+    if (p->ndim != 3) return fail(SMAUG_E_ARG, "the torsional driver (fluxtube1) is 3D");
+    for (int q = 5; q <= 9; q++)
+      if (p->drv[q] == 0) return fail(SMAUG_E_ARG, "the torsional driver needs widths and scales (drv[5..9])");
+  }
+  if (p->driver == SMAUG_DRIVER_SPICULE1 && (p->drv[5] == 0 || p->drv[6] == 0))
+    return fail(SMAUG_E_ARG, "the spicule driver needs widths (drv[5], drv[6])");
   const int np = 2 + 2 * p->ndim;
   for (int f = 0; f < np; f++)
     for (int d = 0; d < p->ndim; d++) {
@@ -224,12 +230,15 @@ static void make_geo(const smaug_params* p, Geo& G) {
   G.terms = p->terms;
   G.hyper = p->hyperdifmom;
   G.driver = p->driver;
-  for (int i = 0; i < 8; i++) G.drv[i] = p->drv[i];
+  for (int i = 0; i < SMAUG_NDRV; i++) G.drv[i] = p->drv[i];
   G.mixed = p->precision == SMAUG_PREC_MIXED;
   // src/divb.cu:165-338 acts in 2D only (its launch covers the k = 0 plane,
   // which its 3D region test excludes)
   G.divb = p->divbon == 1 && p->divbfix != 0 && p->ndim == 2;
-  for (int d = 0; d < 3; d++) G.pdx[d] = d < p->ndim ? p->dx[d] : 1.0;
+  for (int d = 0; d < 3; d++) {
+    G.pdx[d] = d < p->ndim ? p->dx[d] : 1.0;
+    G.pxmin[d] = d < p->ndim ? p->xmin[d] : 0.0;
+  }
 }
 
 // delx along every d depends only on the d coordinate: returns the spacing
@@ -428,9 +437,20 @@ static void run_stage(smaug_t* s, const double* win, const double* base, double*
   s->cslot ^= 1;
 }
 
+// time factor of the driver in the coming step: sin(2 pi qt / P) for the
+// vertical driver; the constant tdep of fluxtube1; spicule1's tdep in the
+// step whose counter p->it is 1 (models/usersource.cu.spicule1:67), else 0
 static double driver_tdep(const smaug_t* s) {
-  if (s->p.driver != SMAUG_DRIVER_VERTICAL) return 0.0;
-  return sin(2.0 * M_PI * s->qt / s->p.drv[1]);
+  switch (s->p.driver) {
+    case SMAUG_DRIVER_VERTICAL:
+      return sin(2.0 * M_PI * s->qt / s->p.drv[1]);
+    case SMAUG_DRIVER_TORSIONAL:
+      return s->p.drv[1];
+    case SMAUG_DRIVER_SPICULE1:
+      return s->it == 1 ? s->p.drv[1] : 0.0;
+    default:
+      return 0.0;
+  }
 }
 
 // One time step.  With a tape (device buffer of smaug_tape_doubles values)

@@ -264,7 +264,8 @@ void launch_dt(hipStream_t st, const unsigned long long* cmax, int nd, const dou
 
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A) { launch_rhs_tiled(G, st, A); }
 
-// per-axis factors of the vertical driver (models/usersource.cu.fluxtube1:3-90):
+// per-axis factors of the vertical driver (the Gaussian of
+// models/usersource.cu.fluxtube1:59-62):
 // the same expressions k_rhs evaluates per cell when the grid is not separable
 __global__ __launch_bounds__(256) void k_drvtab(Geo G, const double* __restrict__ pos1, double* __restrict__ dshape) {
   const int t = blockIdx.x * 256 + threadIdx.x;

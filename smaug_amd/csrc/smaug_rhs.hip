@@ -791,6 +791,33 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
     out[fMOM(0)] += h * vz * rto;
     out[fE(ND)] += h * (vz * vz) * rto / 2.0;
   }
+  // fluxtube1's torsional driver (models/usersource.cu.fluxtube1:3-90), 3D
+  if (ND == 3 && (T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_TORSIONAL) {
+    const double* dv = G.drv;
+    const double xp = at(A.pos + N, c) - dv[3];
+    const double zp = at(A.pos, c) - dv[2];
+    const double yp = at(A.pos + 2 * N, c) - dv[4];
+    const double exz = exp(-zp * zp / (dv[5] * dv[5]));
+    const double exx = exp(-xp * xp / (dv[6] * dv[6]));
+    const double exy = exp(-yp * yp / (dv[7] * dv[7]));
+    const double exyz = exx * exy * exz;
+    const double vx = (dv[0] * yp / dv[9]) * exyz * A.tdep;
+    const double vy = -(dv[0] * xp / dv[8]) * exyz * A.tdep;
+    const double rto = out[fRHO()] + rhob_c;
+    out[fMOM(1)] += h * vx * rto;
+    out[fMOM(2)] += h * vy * rto;
+    out[fE(ND)] += h * (vx * vx + vy * vy) * rto / 2.0;
+  }
+  // spicule1's driver (models/usersource.cu.spicule1:4-136): tdep is 0
+  // outside the step whose counter is 1; grid positions xmin + i dx
+  if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_SPICULE1 && A.tdep != 0.0) {
+    const double* dv = G.drv;
+    const double xp = G.pxmin[0] + (double)x[0] * G.pdx[0];
+    const double yp = G.pxmin[1] + (double)x[1] * G.pdx[1];
+    const double r1 = (xp - dv[2]) * (xp - dv[2]), r2 = (yp - dv[3]) * (yp - dv[3]);
+    const double vvz = dv[0] * (exp(-r2 / (dv[6] * dv[6])) * exp(-r1 / (dv[5] * dv[5]))) * A.tdep;
+    out[fMOM(0)] = out[fMOM(0)] + h * vvz * (out[fRHO()] + rhob_c);
+  }
   if (A.r1) {
 #pragma unroll
     for (int f = 0; f < NP; f++) {

@@ -10,7 +10,7 @@ The reference fixes its parameters at compile time with ``#define``s in
 Recognised keys (reference names): NI NJ NK (interior cells), XMIN XMAX
 YMIN YMAX ZMIN ZMAX, DT, PGAMMA, PMU, PETA, PGRAV0..2, PCOURANT, PRKON,
 PMODDTON, PHYPERDIFMOM, PDIVBON, PCHYP, PCHYPRHO, PCHYPENERGY, PCHYPB0..2,
-PCHYPMOM0..2, BCU0..2, BCL0..2, plus DRIVER and DRV0..DRV7 for the user
+PCHYPMOM0..2, BCU0..2, BCL0..2, plus DRIVER and DRV0..DRV11 for the user
 source plug-in, PRECISION (fp64|mixed) and NDIM.
 """
 from __future__ import annotations
@@ -32,6 +32,9 @@ T_FLUX, T_ETERM, T_GRAV, T_HRHO, T_HE, T_HMOM, T_HB, T_SOURCE = (1 << i for i in
 
 DRIVER_NONE = 0
 DRIVER_VERTICAL = 1
+DRIVER_TORSIONAL = 2  # models/usersource.cu.fluxtube1
+DRIVER_SPICULE1 = 3   # models/usersource.cu.spicule1
+NDRV = 12
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -59,7 +62,7 @@ class SmaugParams(C.Structure):
         ("dx", C.c_double * 3),
         ("boundtype", ((C.c_int * 2) * 3) * 13),
         ("driver", C.c_int),
-        ("drv", C.c_double * 8),
+        ("drv", C.c_double * NDRV),
         ("terms", C.c_uint),
         ("precision", C.c_int),
         ("qt0", C.c_double),
@@ -111,7 +114,7 @@ class RunParams:
     # fields (the reference's boundtype[field][dir] table, include/iotypes.h:144)
     bc_field: Dict[str, List[int]] = field(default_factory=dict)
     driver: int = DRIVER_NONE
-    drv: List[float] = field(default_factory=lambda: [0.0] * 8)
+    drv: List[float] = field(default_factory=lambda: [0.0] * NDRV)
     terms: int = TERMS_ALL
     precision: int = PREC_FP64
     qt0: float = 0.0
@@ -159,7 +162,7 @@ class RunParams:
                 p.boundtype[f][d][0] = (over[d] if over else self.bcu[d]) if d < self.ndim else 0
                 p.boundtype[f][d][1] = (over[d] if over else self.bcl[d]) if d < self.ndim else 0
         p.driver = self.driver
-        for i in range(8):
+        for i in range(NDRV):
             p.drv[i] = self.drv[i] if i < len(self.drv) else 0.0
         p.terms = self.terms
         p.precision = self.precision
@@ -214,7 +217,7 @@ def from_keys(kv: Dict[str, str]) -> RunParams:
         if key in kv:
             chyp[nm] = float(kv[key])
     rp.chyp = chyp
-    rp.drv = [float(kv.get(f"DRV{i}", 0.0)) for i in range(8)]
+    rp.drv = [float(kv.get(f"DRV{i}", 0.0)) for i in range(NDRV)]
     prec = kv.get("PRECISION", "fp64").lower()
     rp.precision = PREC_MIXED if prec == "mixed" else PREC_FP64
     return rp
@@ -249,4 +252,13 @@ def validate(rp: RunParams) -> Optional[str]:
                 return f"boundary type {bt} in dir {d}: the MPI boundary types need a multi-domain run"
             if not 0 <= bt <= 7:
                 return f"unknown boundary type {bt} in dir {d}"
+    dv = list(rp.drv) + [0.0] * NDRV
+    if rp.driver not in (DRIVER_NONE, DRIVER_VERTICAL, DRIVER_TORSIONAL, DRIVER_SPICULE1):
+        return "unknown driver"
+    if rp.driver == DRIVER_VERTICAL and 0.0 in (dv[1], dv[5], dv[6]):
+        return "the vertical driver needs a period and widths (DRV1, DRV5, DRV6)"
+    if rp.driver == DRIVER_TORSIONAL and (rp.ndim != 3 or 0.0 in (dv[5], dv[6], dv[7], dv[8], dv[9])):
+        return "the torsional driver is 3D and needs widths and scales (DRV5..DRV9)"
+    if rp.driver == DRIVER_SPICULE1 and 0.0 in (dv[5], dv[6]):
+        return "the spicule driver needs widths (DRV5, DRV6)"
     return None

@@ -48,6 +48,14 @@ def make_case(name: str, terms: int = P.TERMS_ALL):
         w[4] += 1e-3 * np.sin(2 * np.pi * (z - z.min()) / (z.max() - z.min())) * w[8]
         rp.terms = terms
         return rp, w, pos, None, 7, 3
+    if name in ("tor", "spi", "spe"):  # the torsional (3D RK4) and spicule (2D RK4, Euler) drivers
+        from tests.test_drivers import spicule_case, torsional_case
+
+        if name == "tor":
+            rp, w, pos = torsional_case(rkon=1, terms=terms)
+            return rp, w, pos, None, 6, 2
+        rp, w, pos = spicule_case(ndim=2, rkon=1 if name == "spi" else 0, terms=terms)
+        return rp, w, pos, None, 7, 3
     if name == "s3m":  # 3D RK4 on a grid of several stage-kernel regions and dim-2 chunks
         rp, w, pos = strat_case(ni=(30, 17, 37), ndim=3, rkon=1)
         rp.terms = terms

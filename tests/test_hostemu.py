@@ -136,3 +136,15 @@ def test_short_caller_buffers_are_refused(hostlib):
     assert set(r) == {"save_pert", "load_pert", "step_taped", "adjoint_step", "set_driver_map", "cfl_noalloc"}
     for name, msg in r.items():
         assert "failed (-1)" in msg and "past the end of its allocation" in msg, (name, msg)
+
+
+@pytest.mark.parametrize("case", ["tor", "spi", "spe"])
+def test_reference_drivers_forward_and_adjoint(hostlib, case):
+    """fluxtube1's torsional driver (3D RK4) and spicule1's (2D RK4 and
+    Euler, over the step whose counter is 1): forward parity with the oracle
+    and the adjoint against the replayed differences."""
+    r = run(hostlib, "fwd", case)
+    assert max(r["rel_l2"]) <= 1e-12, r
+    r = run(hostlib, "adj", case)
+    for f, pair in enumerate(r["fields"]):
+        assert close(pair), (f, pair)
