@@ -175,6 +175,16 @@ __device__ inline bool z4(const Geo& G, int idx, int d, int x) {
 __device__ inline bool z1(const Geo& G, int idx, int d, int x) {
   return ((G.zm[idx] >> d) & 1u) && (x == 2 || x == G.n[d] - 3);
 }
+// the same tests where the caller knows whether x can lie on a zeroed
+// layer at all (EDGE false: x in [5, n-6], never zeroed)
+template <bool EDGE>
+__device__ inline bool z4e(const Geo& G, int idx, int d, int x) {
+  return EDGE && z4(G, idx, d, x);
+}
+template <bool EDGE>
+__device__ inline bool z1e(const Geo& G, int idx, int d, int x) {
+  return EDGE && z1(G, idx, d, x);
+}
 
 
 // ---------------------------------------------------------------------------

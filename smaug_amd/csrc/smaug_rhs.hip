@@ -122,7 +122,7 @@ struct CellLoads {
   SH v[NK][ND + 1];
 };
 
-template <int ND, typename SH>
+template <int ND, typename SH, bool EDGE>
 __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
                                   const CellBox<ND, SH>& C, const int* o, CellLoads<ND, SH>& R) {
   using L = Lay<ND>;
@@ -134,7 +134,7 @@ __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const
     int a = l % L::PX, b = (l / L::PX) % L::PY, cz = l / (L::PX * L::PY);
     if (ND == 3 && C.shell && l < Shell<ND>::SHN) Shell<ND>::cell(l, a, b, cz);
     const int x[3] = {o[0] - 1 + a, o[1] - 1 + b, ND == 3 ? o[2] - 1 + cz : 0};
-    g[k] = l < (ND == 3 && C.shell ? Shell<ND>::SHN : L::PN) && in_grid<ND>(G, x) ? (int)cix(G, x[0], x[1], x[2]) : -1;
+    g[k] = l < (ND == 3 && C.shell ? Shell<ND>::SHN : L::PN) && (!EDGE || in_grid<ND>(G, x)) ? (int)cix(G, x[0], x[1], x[2]) : -1;
   }
 #pragma unroll
   for (int k = 0; k < NK; k++) {
@@ -198,19 +198,20 @@ __device__ inline void stage_q_edge(const Geo& G, const SH* __restrict__ qa, SH*
 
 // q box (tile + 3-cell halo) of field qa into qb, with it the cell box C
 // when C.CBX is set and a second q box (qa2 into qb2) when qa2 is set: all
-// of their loads are issued before the first store
-template <int ND, typename SH>
+// of their loads are issued before the first store.  EDGE false: the
+// caller's tile lies at least five cells inside the grid (the box is inside)
+template <int ND, typename SH, bool EDGE>
 __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o,
                                const SH* __restrict__ ws = nullptr, const double* __restrict__ bg = nullptr,
                                CellBox<ND, SH> C = CellBox<ND, SH>{nullptr, false, 0}, const SH* __restrict__ qa2 = nullptr,
                                SH* __restrict__ qb2 = nullptr) {
   using L = Lay<ND>;
   CellLoads<ND, SH> cl;
-  if (C.CBX) cells_load<ND, SH>(G, ws, bg, C, o, cl);
+  if (C.CBX) cells_load<ND, SH, EDGE>(G, ws, bg, C, o, cl);
   // fast path: the halo box lies inside the grid, no substitutions needed;
   // each thread owns (a,b) columns of the box and walks their planes
-  bool inside = o[0] - 3 >= 0 && o[1] - 3 >= 0 && o[0] + L::TX + 3 <= G.n[0] && o[1] + L::TY + 3 <= G.n[1];
-  if (ND == 3) inside = inside && o[2] - 3 >= 0 && o[2] + L::TZ + 3 <= G.n[2];
+  bool inside = !EDGE || (o[0] - 3 >= 0 && o[1] - 3 >= 0 && o[0] + L::TX + 3 <= G.n[0] && o[1] + L::TY + 3 <= G.n[1]);
+  if (EDGE && ND == 3) inside = inside && o[2] - 3 >= 0 && o[2] + L::TZ + 3 <= G.n[2];
   if (inside) {
     constexpr int PL = L::QX * L::QY, NKF = (PL + 255) / 256;
     const unsigned zs = (unsigned)G.n[0] * (unsigned)G.n[1];
@@ -244,51 +245,25 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
     }
     return;
   }
-  if (C.CBX) cells_store<ND, SH>(C, cl);
-  stage_q_edge<ND, SH>(G, qa, qb, o);
-  if (qa2) stage_q_edge<ND, SH>(G, qa2, qb2, o);
+  if constexpr (EDGE) {
+    if (C.CBX) cells_store<ND, SH>(C, cl);
+    stage_q_edge<ND, SH>(G, qa, qb, o);
+    if (qa2) stage_q_edge<ND, SH>(G, qa2, qb2, o);
+  }
 }
 
-// FP32 stencils: at least five wavefronts per SIMD (96 VGPRs; measured
-// faster at 256^3 than four without a spill or six with a 72-byte spill).
-// FP64: three, the number its 52 KB of LDS per workgroup allows (146
-// VGPRs); stating it lets the scheduler spend registers up to that bound
-// (measured 0.8 % faster than a bound of one workgroup).
-template <int ND, typename SH>
-__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo G, RhsArgs A) {
+// The terms of one tile whose origin (first interior cell) is o.  EDGE
+// false: every cell of the tile lies in [5, n-6] along every direction, so
+// no cell of it or of its halos is a ghost or on a zeroed layer, every
+// staged box lies inside the grid and no cell has a ghost image; the
+// boundary checks are then compiled out (the arithmetic is the same).
+template <int ND, typename SH, bool EDGE>
+__device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __restrict__ sm, const int* o) {
   using L = Lay<ND>;
   using AT = SH;  // arithmetic type of the stencils (the own-cell update and the RK combination are FP64)
   constexpr int NP = L::NP;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
-  __shared__ AT sm[L::SMEM];
   const int tid = threadIdx.x;
-  if (A.zero_next && blockIdx.x == 0 && tid < 3) A.zero_next[tid] = 0ull;
-  // XCD-aware tile order: the hardware deals consecutive workgroups to the
-  // 8 XCDs in turn, so workgroup b runs on XCD b%8.  In 3D XCD x takes the
-  // slab of tile layers [x*gz/8, (x+1)*gz/8) and walks it x fastest, then
-  // z, then y, so the z neighbour of a tile runs gx tiles after it (its
-  // halo planes still in that XCD's L2) and the y neighbour gx*gz/8 tiles
-  // after it (measured 1.7 % faster at 256^3 than x, y, z order, whose z
-  // neighbour comes gx*gy tiles later); otherwise XCD x takes the tiles
-  // [x*nb/8, (x+1)*nb/8) in x, y, z order.
-  const int gx = (G.n[0] - 4 + TX - 1) / TX, gy = (G.n[1] - 4 + TY - 1) / TY;
-  const int gz = ND == 3 ? (G.n[2] - 4 + TZ - 1) / TZ : 1;
-  const int nb = gx * gy * gz;
-  const int b = blockIdx.x;
-  int t = b;
-  if ((nb & 7) == 0) {
-    const int xcd = b & 7, tl = b >> 3;
-    if (ND == 3 && (gz & 7) == 0) {
-      const int zc = gz >> 3;  // tile layers per XCD
-      const int ux = tl % gx, r = tl / gx;
-      const int uz = xcd * zc + r % zc, uy = r / zc;
-      t = (uz * gy + uy) * gx + ux;
-    } else {
-      t = xcd * (nb >> 3) + tl;
-    }
-  }
-  const int tbx = t % gx, tby = (t / gx) % gy, tbz = t / (gx * gy);
-  const int o[3] = {2 + tbx * TX, 2 + tby * TY, ND == 3 ? 2 + tbz * TZ : 0};
   const int lc[3] = {tid % TX, (tid / TX) % TY, ND == 3 ? tid / (TX * TY) : 0};
   int x[3] = {o[0] + lc[0], o[1] + lc[1], o[2] + lc[2]};
   const bool valid = interior_c<ND>(G, x);
@@ -399,7 +374,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
         for (int f = 0; f < NP; f++) FB[f * L::FN + l1] = F[f];
       }
       __syncthreads();
-      if (valid && !z4(G, zFLUX(ND), d, x[d])) {
+      if (valid && !z4e<EDGE>(G, zFLUX(ND), d, x[d])) {
         const int s = d == 0 ? 1 : (d == 1 ? rx : rx * ry);
         const int li = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
         const AT inv = rdx[d] * AT(1.0 / 12.0);
@@ -436,7 +411,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
   if (h_rho || h_e) {
     const SH* qe = pr + (long long)pQE(ND) * N;
     SH* qbe = h_rho ? qb + L::QN : qb;
-    stage_q<ND, SH>(G, h_rho ? ws : qe, qb, o, ws, bg, CellBox<ND, SH>{nullptr, false, 0}, h_rho && h_e ? qe : nullptr,
+    stage_q<ND, SH, EDGE>(G, h_rho ? ws : qe, qb, o, ws, bg, CellBox<ND, SH>{nullptr, false, 0}, h_rho && h_e ? qe : nullptr,
                     h_rho && h_e ? qbe : nullptr);
     __syncthreads();
     if (valid && h_rho) {
@@ -444,7 +419,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
       for (int dim = 0; dim < ND; dim++) {
         AT nur, nul;
         nu_win<SH, AT>(qb, qc, qs[dim], dx[dim] * cm[dim] * (AT)G.chyp[fRHO()], nur, nul);
-        const bool z = z1(G, zRHO, dim, x[dim]);
+        const bool z = z1e<EDGE>(G, zRHO, dim, x[dim]);
         const AT q0 = qb[qc];
         const AT gr = z ? AT(0) : (qb[qc + qs[dim]] - q0) * rdx[dim];
         const AT gl = z ? AT(0) : (q0 - qb[qc - qs[dim]]) * rdx[dim];
@@ -456,7 +431,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
       for (int dim = 0; dim < ND; dim++) {
         AT nur, nul;
         nu_win<SH, AT>(qbe, qc, qs[dim], dx[dim] * cm[dim] * (AT)G.chyp[fE(ND)], nur, nul);
-        const bool z = z1(G, zTMP1, dim, x[dim]);
+        const bool z = z1e<EDGE>(G, zTMP1, dim, x[dim]);
         const AT q0 = qbe[qc];
         const AT gr = z ? AT(0) : (qbe[qc + qs[dim]] - q0) * rdx[dim];
         const AT gl = z ? AT(0) : (q0 - qbe[qc - qs[dim]]) * rdx[dim];
@@ -483,7 +458,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
 #pragma unroll
     for (int f = 0; f < ND; f++) {
       // the first box brings rho_t and m on the cell box with it
-      stage_q<ND, SH>(G, pr + (long long)pV(f) * N, qb, o, ws, bg,
+      stage_q<ND, SH, EDGE>(G, pr + (long long)pV(f) * N, qb, o, ws, bg,
                       CellBox<ND, SH>{f == 0 && want_mom ? CBX : nullptr, true, fMOM(0), own_cb});
       __syncthreads();
       if (want_eterm && valid) {  // addenergyterms_cd2 for dir f
@@ -491,7 +466,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
 #pragma unroll
         for (int i2 = 0; i2 < ND; i2++) {
           const int s = qs[i2];
-          gf[i2] = z4(G, zVEL(f), i2, x[i2])
+          gf[i2] = z4e<EDGE>(G, zVEL(f), i2, x[i2])
                        ? AT(0)
                        : ((8 * qb[qc + s] - 8 * qb[qc - s]) + (qb[qc - 2 * s] - qb[qc + 2 * s])) * (rdx[i2] * AT(1.0 / 12.0));
         }
@@ -520,7 +495,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
             AT g7 = 0, g8 = 0;
             if (valid) {
               const AT t5 =
-                  z1(G, zTMP4, d, x[d]) ? AT(0) : (qb[qc + qs[d]] - qb[qc - qs[d]]) * rdx[d] * AT(0.5);
+                  z1e<EDGE>(G, zTMP4, d, x[d]) ? AT(0) : (qb[qc + qs[d]] - qb[qc - qs[d]]) * rdx[d] * AT(0.5);
               const AT t6 = t5 * (nur[d] + nul[d]) * AT(0.25);
               g7 = rt_c * t6;
               g8 = t6 * CBX[(1 + d) * L::PN + pc];
@@ -568,7 +543,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
                 AT nr, nl;
                 nu_win<SH, AT>(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
                 const AT t5 =
-                    z1(G, zTMP4, d, xr[d]) ? AT(0) : (qb[qi + qs[d]] - qb[qi - qs[d]]) * rdxr * AT(0.5);
+                    z1e<EDGE>(G, zTMP4, d, xr[d]) ? AT(0) : (qb[qi + qs[d]] - qb[qi - qs[d]]) * rdxr * AT(0.5);
                 const AT t6 = t5 * (nr + nl) * AT(0.25);
                 g7 = rt * t6;
                 g8 = t6 * CBX[(1 + d) * L::PN + pi];
@@ -586,7 +561,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
           for (int d = 0; d < ND; d++) {
             const int s = qs[d];
             const int sp = ps[d];
-            const bool z = z1(G, zTMP4, d, x[d]);
+            const bool z = z1e<EDGE>(G, zTMP4, d, x[d]);
             const AT q0 = qb[qc];
             const AT gr = z ? AT(0) : (qb[qc + s] - q0) * rdx[d];
             const AT gl = z ? AT(0) : (q0 - qb[qc - s]) * rdx[d];
@@ -604,8 +579,8 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
               const int k = d < f ? d : d - 1;
               const AT* GAk = RG[2 * k];
               const AT* GBk = RG[2 * k + 1];
-              const AT o7 = z1(G, zTMP7, f, x[f]) ? AT(0) : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * AT(0.5));
-              const AT o8 = z1(G, zTMP8, f, x[f]) ? AT(0) : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * AT(0.5));
+              const AT o7 = z1e<EDGE>(G, zTMP7, f, x[f]) ? AT(0) : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * AT(0.5));
+              const AT o8 = z1e<EDGE>(G, zTMP8, f, x[f]) ? AT(0) : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * AT(0.5));
               acc[fMOM(d)] += o7;
               acc[fE(ND)] += o8;
             }
@@ -624,7 +599,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
     for (int f = 0; f < ND; f++) {
       // the first box brings b on the cell box with it (the momentum
       // phases are done with the cell box: each ended with a barrier)
-      stage_q<ND, SH>(G, ws + (long long)fB(ND, f) * N, qb, o, ws, bg,
+      stage_q<ND, SH, EDGE>(G, ws + (long long)fB(ND, f) * N, qb, o, ws, bg,
                       CellBox<ND, SH>{f == 0 ? CBX : nullptr, false, fB(ND, 0)});
       __syncthreads();
       const int ex[3] = {f == 0 ? 1 : 0, f == 1 ? 1 : 0, f == 2 ? 1 : 0};
@@ -642,7 +617,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
         if (valid) {
           nu_win<SH, AT>(qb, qc, qs[d], dx[d] * cm[d] * Kc, nur[d], nul[d]);
           const AT t2 =
-              z1(G, zTMP1, d, x[d]) ? AT(0) : (qb[qc + qs[d]] - qb[qc - qs[d]]) * rdx[d] * AT(0.5);
+              z1e<EDGE>(G, zTMP1, d, x[d]) ? AT(0) : (qb[qc + qs[d]] - qb[qc - qs[d]]) * rdx[d] * AT(0.5);
           const AT t3 = t2 * (nul[d] + nur[d]) * AT(0.5);
           g3 = t3;
           g5 = t3 * CBX[(1 + d) * L::PN + pc];
@@ -688,7 +663,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
               AT nr, nl;
               nu_win<SH, AT>(qb, qi, qs[d], dxr * cm[d] * Kc, nr, nl);
               const AT t2 =
-                  z1(G, zTMP1, d, xr[d]) ? AT(0) : (qb[qi + qs[d]] - qb[qi - qs[d]]) * rdxr * AT(0.5);
+                  z1e<EDGE>(G, zTMP1, d, xr[d]) ? AT(0) : (qb[qi + qs[d]] - qb[qi - qs[d]]) * rdxr * AT(0.5);
               const AT t3 = t2 * (nl + nr) * AT(0.5);
               g3 = t3;
               g5 = t3 * CBX[(1 + d) * L::PN + pi];
@@ -706,12 +681,12 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
           const int k = d < f ? d : d - 1;
           const AT* GAk = RG[2 * k];
           const AT* GBk = RG[2 * k + 1];
-          const AT o3 = z1(G, zTMP3, f, x[f]) ? AT(0) : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * AT(0.5));
-          const AT o5 = z1(G, zTMP5, f, x[f]) ? AT(0) : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * AT(0.5));
+          const AT o3 = z1e<EDGE>(G, zTMP3, f, x[f]) ? AT(0) : (GAk[ri + sf] - GAk[ri - sf]) * (rdx[f] * AT(0.5));
+          const AT o5 = z1e<EDGE>(G, zTMP5, f, x[f]) ? AT(0) : (GBk[ri + sf] - GBk[ri - sf]) * (rdx[f] * AT(0.5));
           acc[fB(ND, d)] += -o3;
           acc[fE(ND)] += -o5;
           const int s = qs[d];
-          const bool z = z1(G, zTMP1, d, x[d]);
+          const bool z = z1e<EDGE>(G, zTMP1, d, x[d]);
           const AT q0 = qb[qc];
           const AT gr = z ? AT(0) : (qb[qc + s] - q0) * rdx[d];
           const AT gl = z ? AT(0) : (q0 - qb[qc - s]) * rdx[d];
@@ -833,6 +808,7 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
   // indexed at run time; waves away from the domain edge skip the images.
 #pragma unroll
   for (int q = 0; q < NP; q++) put(wo + (long long)q * N, c, out[q]);
+  if (!EDGE) return;  // no cell of an inner tile has a ghost image
 #pragma unroll
   for (int f = 0; f < NP; f++) {
     if (f > 0 && G.bc_uniform) break;
@@ -865,6 +841,55 @@ __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo 
           }
         }
   }
+}
+
+// FP32 stencils: at least five wavefronts per SIMD (96 VGPRs; measured
+// faster at 256^3 than four without a spill or six with a 72-byte spill).
+// FP64: three, the number its 52 KB of LDS per workgroup allows (146
+// VGPRs); stating it lets the scheduler spend registers up to that bound
+// (measured 0.8 % faster than a bound of one workgroup).
+template <int ND, typename SH>
+__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo G, RhsArgs A) {
+  using L = Lay<ND>;
+  constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
+  __shared__ SH sm[L::SMEM];
+  const int tid = threadIdx.x;
+  if (A.zero_next && blockIdx.x == 0 && tid < 3) A.zero_next[tid] = 0ull;
+  // XCD-aware tile order: the hardware deals consecutive workgroups to the
+  // 8 XCDs in turn, so workgroup b runs on XCD b%8.  In 3D XCD x takes the
+  // slab of tile layers [x*gz/8, (x+1)*gz/8) and walks it x fastest, then
+  // z, then y, so the z neighbour of a tile runs gx tiles after it (its
+  // halo planes still in that XCD's L2) and the y neighbour gx*gz/8 tiles
+  // after it (measured 1.7 % faster at 256^3 than x, y, z order, whose z
+  // neighbour comes gx*gy tiles later); otherwise XCD x takes the tiles
+  // [x*nb/8, (x+1)*nb/8) in x, y, z order.
+  const int gx = (G.n[0] - 4 + TX - 1) / TX, gy = (G.n[1] - 4 + TY - 1) / TY;
+  const int gz = ND == 3 ? (G.n[2] - 4 + TZ - 1) / TZ : 1;
+  const int nb = gx * gy * gz;
+  const int b = blockIdx.x;
+  int t = b;
+  if ((nb & 7) == 0) {
+    const int xcd = b & 7, tl = b >> 3;
+    if (ND == 3 && (gz & 7) == 0) {
+      const int zc = gz >> 3;  // tile layers per XCD
+      const int ux = tl % gx, r = tl / gx;
+      const int uz = xcd * zc + r % zc, uy = r / zc;
+      t = (uz * gy + uy) * gx + ux;
+    } else {
+      t = xcd * (nb >> 3) + tl;
+    }
+  }
+  const int tbx = t % gx, tby = (t / gx) % gy, tbz = t / (gx * gy);
+  const int o[3] = {2 + tbx * TX, 2 + tby * TY, ND == 3 ? 2 + tbz * TZ : 0};
+  // inner tile: every cell in [5, n-6] along every direction (the test is
+  // uniform per workgroup; the first and last tile layer along an axis are
+  // edge tiles)
+  bool edge = o[0] < 5 || o[0] + TX - 1 > G.n[0] - 6 || o[1] < 5 || o[1] + TY - 1 > G.n[1] - 6;
+  if (ND == 3) edge = edge || o[2] < 5 || o[2] + TZ - 1 > G.n[2] - 6;
+  if (edge)
+    rhs_tile<ND, SH, true>(G, A, sm, o);
+  else
+    rhs_tile<ND, SH, false>(G, A, sm, o);
 }
 
 template <int ND, typename SH>

@@ -22,6 +22,7 @@
 
 #define __global__
 #define __device__
+#define __forceinline__ inline
 #define __host__
 #define __launch_bounds__(...)
 #define __shared__ static
