@@ -221,12 +221,12 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
       const int l = threadIdx.x + 256 * k;
       if (l < PL) {
         const int a = l % L::QX, b = l / L::QX;
-        const long long g0 = gix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
+        const unsigned g0 = cix(G, o[0] - 3 + a, o[1] - 3 + b, ND == 3 ? o[2] - 3 : 0);
 #pragma unroll
-        for (int cz = 0; cz < L::QZ; cz++) v[k][cz] = qa[g0 + cz * (long long)zs];
+        for (int cz = 0; cz < L::QZ; cz++) v[k][cz] = at(qa, g0 + cz * zs);
         if (qa2) {
 #pragma unroll
-          for (int cz = 0; cz < L::QZ; cz++) v2[k][cz] = qa2[g0 + cz * (long long)zs];
+          for (int cz = 0; cz < L::QZ; cz++) v2[k][cz] = at(qa2, g0 + cz * zs);
         }
       }
     }
@@ -256,7 +256,10 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
 // false: every cell of the tile lies in [5, n-6] along every direction, so
 // no cell of it or of its halos is a ghost or on a zeroed layer, every
 // staged box lies inside the grid and no cell has a ghost image; the
-// boundary checks are then compiled out (the arithmetic is the same).
+// boundary checks are then compiled out (the arithmetic is the same).  The
+// own cell's interior test stays a run-time test in both bodies: known true,
+// it lets the compiler keep values loaded in one phase live through later
+// ones, and the FP64 kernel then spills (336 bytes per lane at gfx950).
 template <int ND, typename SH, bool EDGE>
 __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __restrict__ sm, const int* o) {
   using L = Lay<ND>;
@@ -316,7 +319,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     // the thread's own tile cell: its state is loaded once for every
     // direction (and the flux function's v, p_t, ... of it are shared); a
     // second slot per direction covers the 2 + 2 halo layers along it
-    const bool in0 = in_grid<ND>(G, x);
+    const bool in0 = !EDGE || in_grid<ND>(G, x);
     CellState<ND, AT> S0;
     if (in0) load_cell<ND, SH, AT>(G, ws, bg, c, S0);
     if (own_cb) {  // the tile's cells of the momentum phases' cell box: rho_t, m
@@ -346,7 +349,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
         rc[pa] = r % Tp;
         if (ND == 3) rc[qa] = r / Tp;
         const int xr[3] = {o[0] - ex[0] + rc[0], o[1] - ex[1] + rc[1], ND == 3 ? o[2] - ex[2] + rc[2] : 0};
-        in1 = in_grid<ND>(G, xr);
+        in1 = !EDGE || in_grid<ND>(G, xr);
         l1 = rc[0] + rx * (rc[1] + ry * rc[2]);
         if (in1) load_cell<ND, SH, AT>(G, ws, bg, cix(G, xr[0], xr[1], xr[2]), S1);
       }
@@ -524,7 +527,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
             const int l = bc[0] + rx * (bc[1] + ry * bc[2]);
             const int rel[3] = {bc[0] - ex[0], bc[1] - ex[1], ND == 3 ? bc[2] - ex[2] : 0};
             int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
-            const bool in = interior_c<ND>(G, xr);
+            const bool in = !EDGE || interior_c<ND>(G, xr);
             const unsigned gq = in ? cix(G, xr[0], xr[1], xr[2]) : 0u;
             const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
             const AT rt = in ? CBX[pi] : AT(0);
@@ -645,7 +648,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
           const int l = bc[0] + rx * (bc[1] + ry * bc[2]);
           const int rel[3] = {bc[0] - ex[0], bc[1] - ex[1], ND == 3 ? bc[2] - ex[2] : 0};
           int xr[3] = {o[0] + rel[0], o[1] + rel[1], ND == 3 ? o[2] + rel[2] : 0};
-          const bool in = interior_c<ND>(G, xr);
+          const bool in = !EDGE || interior_c<ND>(G, xr);
           const unsigned gq = in ? cix(G, xr[0], xr[1], xr[2]) : 0u;
           const int pi = (rel[0] + 1) + L::PX * ((rel[1] + 1) + L::PY * (ND == 3 ? rel[2] + 1 : 0));
           const int qi = (rel[0] + 3) + L::QX * ((rel[1] + 3) + L::QY * (ND == 3 ? rel[2] + 3 : 0));
