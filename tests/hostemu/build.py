@@ -19,10 +19,11 @@ def build() -> str:
         os.path.join(HERE, "emu.cpp"), os.path.join(HERE, "hip", "hip_runtime.h")]
     if os.path.exists(OUT) and os.path.getmtime(OUT) >= max(os.path.getmtime(d) for d in deps):
         return OUT
-    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-I", HERE, "-o", OUT + ".tmp",
+    tmp = f"{OUT}.{os.getpid()}.tmp"  # parallel test workers may build at once
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-I", HERE, "-o", tmp,
            "-x", "c++", os.path.join(HERE, "emu.cpp"), *srcs, os.path.join(CSRC, "smaug_io.cpp")]
     subprocess.run(cmd, check=True)
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(tmp, OUT)
     return OUT
 
 
