@@ -267,6 +267,28 @@ int smaug_step_taped(smaug_t* s, double* tape_device);
 int smaug_adjoint_step(smaug_t* s, const double* tape_device, const double* wnext_device, double* lam_device,
                        double* gmap_device);
 
+/* ---- tangent-linear (forward-mode derivative) run -------------------------
+ * No counterpart in the reference: the FP64 tangent-linear model of the
+ * step, the forward-mode derivative of the differentiable binding and the
+ * reference its reverse mode is checked against.  The stage kernel runs on
+ * (value, tangent) pairs: the values follow smaug_step from the context's
+ * current state, the tangents its derivative, with the hyperdiffusion
+ * coefficients, the wave-speed maxima and dt held at their values along that
+ * trajectory.  The run has its own buffers and clock: the context's state and
+ * clock do not change.  Host arrays.  divbon = 1 is rejected.
+ *   smaug_tl_begin -- starts a run from the current state in the direction
+ *                     dw0 (np*N doubles, field-major; its ghost cells are
+ *                     re-filled with the boundary maps) and, when a driver
+ *                     amplitude map is set, damap (n1*n2 values, NULL = 0).
+ *   smaug_tl_step  -- advances the run by nsteps steps.
+ *   smaug_tl_get   -- copies the run's state (w, np*N) and its tangent (dw,
+ *                     np*N); either may be NULL.
+ *   smaug_tl_end   -- frees the run (smaug_destroy does too). */
+int smaug_tl_begin(smaug_t* s, const double* dw0_host, const double* damap_host);
+int smaug_tl_step(smaug_t* s, int nsteps);
+int smaug_tl_get(smaug_t* s, double* w_host, double* dw_host);
+int smaug_tl_end(smaug_t* s);
+
 /* Kernel timing hooks for the benchmark: launches `reps` RK stages of the
  * configured step on the current state and reports the mean device time of
  * each kernel family measured with HIP events on the context's stream. */

@@ -309,6 +309,15 @@ constexpr int kNaCmax = 0, kNaFlag = 3, kNaDts = 4, kNoallocSlots = 6;
 void launch_noalloc_init(hipStream_t st, unsigned long long* s, double dt0);
 void launch_noalloc_out(hipStream_t st, const unsigned long long* s, int nd, double* out);
 void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
+// Tangent-linear step (smaug_tl_*): the stage kernel and k_prim on Dual
+// (value, tangent) arrays, with nu and the wave-speed maxima frozen
+struct Dual;
+void launch_rhs_tl(const Geo& G, hipStream_t st, const RhsArgs& A);
+void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const double* bg, Dual* prim,
+                    unsigned long long* cmax, int* flag);
+// d[i] = (v[i], t[i]) (t null: tangent 0), and back (either output may be null)
+void launch_dual_pack(hipStream_t st, long long n, const double* v, const double* t, Dual* d);
+void launch_dual_unpack(hipStream_t st, long long n, const Dual* d, double* v, double* t);
 void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A);
 void launch_fill(const Geo& G, hipStream_t st, double* w);
 // the per-axis factors of the vertical driver's shape from per-axis

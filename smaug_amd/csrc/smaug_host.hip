@@ -11,6 +11,7 @@
 
 #include "../../include/smaug_hip.h"
 #include "smaug_dev.h"
+#include "smaug_ops.h"
 
 
 using namespace smaug;
@@ -96,7 +97,34 @@ struct smaug_ctx {
   double* amap = nullptr;    // driver amplitude per (j,k) column
   double* adj = nullptr;     // adjoint work: lam_s1..3, mu, T (np*N each), gcell (N)
   double* aprim = nullptr;   // FP64 k_prim of an adjoint stage input
+  // tangent-linear run (smaug_tl_*): Dual stage buffers, derived fields
+  // and driver map, its own maxima slots, (dt, t), clock and host staging
+  Dual* tlU[4] = {nullptr, nullptr, nullptr, nullptr};
+  Dual* tlprim = nullptr;
+  Dual* tlamap = nullptr;
+  unsigned long long* tlcmax = nullptr;  // 2 slots of 4
+  double* tldts = nullptr;
+  double* tltmp = nullptr;  // 2 np*N doubles
+  int tlcur = 0, tlslot = 0, tlit = 0;
+  double tlqt = 0;
+  bool tl = false;
 };
+
+static void free_tl(smaug_t* s) {
+  for (auto& u : s->tlU) {
+    hipFree(u);
+    u = nullptr;
+  }
+  hipFree(s->tlprim);
+  hipFree(s->tlamap);
+  hipFree(s->tlcmax);
+  hipFree(s->tldts);
+  hipFree(s->tltmp);
+  s->tlprim = s->tlamap = nullptr;
+  s->tlcmax = nullptr;
+  s->tldts = s->tltmp = nullptr;
+  s->tl = false;
+}
 
 // ---- grid: positions extrapolated past the interior, delx (initgrid,
 // src/init.cu:2502-3051, quirk Q11) -----------------------------------------
@@ -278,6 +306,7 @@ static void free_ctx(smaug_t* s) {
   hipFree(s->amap);
   hipFree(s->adj);
   hipFree(s->aprim);
+  free_tl(s);
   if (s->own) hipStreamDestroy(s->own);
   delete s;
 }
@@ -440,18 +469,19 @@ static void run_stage(smaug_t* s, const double* win, const double* base, double*
 // time factor of the driver in the coming step: sin(2 pi qt / P) for the
 // vertical driver; the constant tdep of fluxtube1; spicule1's tdep in the
 // step whose counter p->it is 1 (models/usersource.cu.spicule1:67), else 0
-static double driver_tdep(const smaug_t* s) {
+static double driver_tdep_at(const smaug_t* s, int it, double qt) {
   switch (s->p.driver) {
     case SMAUG_DRIVER_VERTICAL:
-      return sin(2.0 * M_PI * s->qt / s->p.drv[1]);
+      return sin(2.0 * M_PI * qt / s->p.drv[1]);
     case SMAUG_DRIVER_TORSIONAL:
       return s->p.drv[1];
     case SMAUG_DRIVER_SPICULE1:
-      return s->it == 1 ? s->p.drv[1] : 0.0;
+      return it == 1 ? s->p.drv[1] : 0.0;
     default:
       return 0.0;
   }
 }
+static double driver_tdep(const smaug_t* s) { return driver_tdep_at(s, s->it, s->qt); }
 
 // One time step.  With a tape (device buffer of smaug_tape_doubles values)
 // the stage inputs are written into it (w^n, then the three intermediate
@@ -866,6 +896,143 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
     launch_colsum(s->G, st, gcell, gmap);
     return launched("k_colsum");
   }
+  return SMAUG_OK;
+}
+
+// ---- tangent-linear run --------------------------------------------------
+// The stage kernel and k_prim instantiated on Dual (value, tangent) arrays:
+// the values follow the forward step from the context's current state, the
+// tangents its derivative in the direction given at the start, with the
+// hyperdiffusion coefficients, the wave-speed maxima and dt frozen at their
+// values (the derivative the adjoint takes).  Own buffers, own maxima slots
+// and clock: the context's state and clock are left as they are.
+
+extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* damap) {
+  if (!s || !dw0) return fail(SMAUG_E_ARG, "null argument");
+  if (s->G.divb) return fail(SMAUG_E_UNSUPPORTED, "the tangent-linear step does not include the divB source terms");
+  if (damap && !s->amap) return fail(SMAUG_E_ARG, "a driver-map direction needs a driver amplitude map (smaug_set_driver_map)");
+  if ((unsigned long long)s->N * sizeof(Dual) >= (1ull << 32))
+    return fail(SMAUG_E_ARG, "the tangent-linear step holds fewer than 2^28 cells");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  free_tl(s);
+  const long long F = (long long)s->np * s->N;
+  const long long ncol = s->N / s->G.n[0];
+  const int nbuf = s->p.rkon == 1 ? 4 : 2;
+  bool ok = true;
+  for (int b = 0; b < nbuf; b++) ok = ok && hipMalloc(&s->tlU[b], sizeof(Dual) * (size_t)F) == hipSuccess;
+  ok = ok && hipMalloc(&s->tlprim, sizeof(Dual) * (size_t)(s->npr * s->N)) == hipSuccess;
+  if (s->amap) ok = ok && hipMalloc(&s->tlamap, sizeof(Dual) * (size_t)ncol) == hipSuccess;
+  ok = ok && hipMalloc(&s->tlcmax, sizeof(unsigned long long) * 8) == hipSuccess;
+  ok = ok && hipMalloc(&s->tldts, sizeof(double) * 2) == hipSuccess;
+  ok = ok && hipMalloc(&s->tltmp, sizeof(double) * (size_t)(2 * F)) == hipSuccess;
+  if (!ok) {
+    free_tl(s);
+    return fail(SMAUG_E_HIP, "tangent-linear buffers: allocation failed");
+  }
+  hipStream_t st = s->stream;
+  // the direction with its ghost images (the maps are linear; a fixed
+  // ghost keeps the value given), then value and tangent interleaved
+  HIPCHK(hipMemcpyAsync(s->tltmp, dw0, sizeof(double) * (size_t)F, hipMemcpyHostToDevice, st));
+  Geo Gt = s->G;
+  Gt.mixed = 0;
+  launch_fill(Gt, st, s->tltmp);
+  launch_dual_pack(st, F, s->U[s->cur], s->tltmp, s->tlU[0]);
+  for (int b = 1; b < nbuf; b++)  // fixed ghosts of every stage buffer hold the initial values
+    HIPCHK(hipMemcpyAsync(s->tlU[b], s->tlU[0], sizeof(Dual) * (size_t)F, hipMemcpyDeviceToDevice, st));
+  if (s->tlamap) {
+    double* dm = s->tltmp + F;
+    if (damap)
+      HIPCHK(hipMemcpyAsync(dm, damap, sizeof(double) * (size_t)ncol, hipMemcpyHostToDevice, st));
+    launch_dual_pack(st, ncol, s->amap, damap ? dm : nullptr, s->tlamap);
+  }
+  launch_zero3(st, s->tlcmax);
+  launch_zero3(st, s->tlcmax + 4);
+  HIPCHK(hipMemcpyAsync(s->tldts, s->dts, sizeof(double) * 2, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));  // the host arrays may go once this returns
+  s->tlcur = 0;
+  s->tlslot = 0;
+  s->tlit = s->it;
+  s->tlqt = s->qt;
+  s->tl = true;
+  return SMAUG_OK;
+}
+
+static void tl_stage(smaug_t* s, const Geo& Gt, Dual* win, const Dual* base, Dual* wout, double hfrac, double tdep,
+                     bool first, const Dual* r1, const Dual* r2, const Dual* r3) {
+  hipStream_t st = s->stream;
+  const int slot = s->tlslot;
+  launch_prim_tl(Gt, st, win, s->bg, s->tlprim, s->tlcmax + 4 * slot, s->flag);
+  if (first)
+    launch_dt(st, s->tlcmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->tldts);
+  RhsArgs A;
+  A.win = (const double*)win;
+  A.wsh = win;
+  A.bg = s->bg;
+  A.prim = s->tlprim;
+  A.delx = s->delx;
+  A.rdelx = s->rdelx;
+  A.dx1 = s->dx1;
+  A.rdx1 = s->rdx1;
+  A.pos = s->pos;
+  A.base = (const double*)base;
+  A.r1 = (const double*)r1;
+  A.r2 = (const double*)r2;
+  A.r3 = (const double*)r3;
+  A.wout = (double*)wout;
+  A.cmax_bits = s->tlcmax + 4 * slot;
+  A.zero_next = s->tlcmax + 4 * (slot ^ 1);
+  A.dts = s->tldts;
+  A.hfrac = hfrac;
+  A.tdep = tdep;
+  A.amap = (const double*)s->tlamap;
+  A.dshape = s->dshape;
+  launch_rhs_tl(Gt, st, A);
+  s->tlslot ^= 1;
+}
+
+extern "C" int smaug_tl_step(smaug_t* s, int nsteps) {
+  if (!s || nsteps < 0) return fail(SMAUG_E_ARG, "bad arguments to smaug_tl_step");
+  if (!s->tl) return fail(SMAUG_E_ARG, "no tangent-linear run (smaug_tl_begin)");
+  Geo Gt = s->G;
+  Gt.mixed = 0;
+  for (int n = 0; n < nsteps; n++) {
+    s->tlqt = s->tlqt + s->p.dt;  // as the forward step (quirk Q5)
+    const double tdep = driver_tdep_at(s, s->tlit, s->tlqt);
+    if (s->p.rkon == 0) {
+      tl_stage(s, Gt, s->tlU[s->tlcur], s->tlU[s->tlcur], s->tlU[s->tlcur ^ 1], 1.0, tdep, true, nullptr, nullptr,
+               nullptr);
+      s->tlcur ^= 1;
+    } else {
+      Dual *w0 = s->tlU[0], *w1 = s->tlU[1], *w2 = s->tlU[2], *w3 = s->tlU[3];
+      tl_stage(s, Gt, w0, w0, w1, 0.5, tdep, true, nullptr, nullptr, nullptr);
+      tl_stage(s, Gt, w1, w0, w2, 0.5, tdep, false, nullptr, nullptr, nullptr);
+      tl_stage(s, Gt, w2, w0, w3, 1.0, tdep, false, nullptr, nullptr, nullptr);
+      tl_stage(s, Gt, w3, w0, w0, 1.0 / 6.0, tdep, false, w1, w2, w3);
+    }
+    s->tlit = s->tlit + 1;
+    HIPCHK(hipGetLastError());
+  }
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_tl_get(smaug_t* s, double* w, double* dw) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  if (!s->tl) return fail(SMAUG_E_ARG, "no tangent-linear run (smaug_tl_begin)");
+  const long long F = (long long)s->np * s->N;
+  hipStream_t st = s->stream;
+  launch_dual_unpack(st, F, s->tlU[s->tlcur], s->tltmp, s->tltmp + F);
+  HIPCHK(hipGetLastError());
+  if (w) HIPCHK(hipMemcpyAsync(w, s->tltmp, sizeof(double) * (size_t)F, hipMemcpyDeviceToHost, st));
+  if (dw) HIPCHK(hipMemcpyAsync(dw, s->tltmp + F, sizeof(double) * (size_t)F, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return check_flag(s);
+}
+
+extern "C" int smaug_tl_end(smaug_t* s) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  free_tl(s);
   return SMAUG_OK;
 }
 

@@ -17,6 +17,7 @@
 
 #include "../../include/smaug_hip.h"
 #include "smaug_dev.h"
+#include "smaug_ops.h"
 
 namespace smaug {
 
@@ -131,6 +132,102 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
   }
 }
 
+// k_prim of the tangent-linear step: w holds Dual values (value and
+// tangent of every evolved field); v and q_e are formed in Dual arithmetic
+// (the same expressions as k_prim), the wave speeds and their maxima from
+// the values alone (frozen, as the hyperdiffusion coefficients they scale)
+template <int ND>
+__global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__ w, const double* __restrict__ bg,
+                                                 Dual* __restrict__ prim, unsigned long long* __restrict__ cmax_bits,
+                                                 int* __restrict__ flag) {
+  const long long N = G.N;
+  double mx[3] = {0.0, 0.0, 0.0};
+  int bad = 0;
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += stride) {
+    const Dual rho = w[c], e = w[(long long)fE(ND) * N + c];
+    const double rhob = bg[(long long)bRHOB() * N + c], eb = bg[(long long)bEB() * N + c];
+    Dual m[ND], b[ND];
+    double bb[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      m[d] = w[(long long)fMOM(d) * N + c];
+      b[d] = w[(long long)fB(ND, d) * N + c];
+      bb[d] = bg[(long long)bBB(d) * N + c];
+    }
+    const Dual rt = rho + rhob;
+    Dual mm = m[0] * m[0], b2 = b[0] * b[0];
+#pragma unroll
+    for (int d = 1; d < ND; d++) {
+      mm = mm + m[d] * m[d];
+      b2 = b2 + b[d] * b[d];
+    }
+#pragma unroll
+    for (int d = 0; d < ND; d++) prim[(long long)pV(d) * N + c] = m[d] / rt;
+    prim[(long long)pQE(ND) * N + c] = e - 0.5 * (b2 + mm / rt);
+    // wave speeds of the values (k_prim's expressions)
+    const double gm = G.gamma, rtv = rt.v;
+    double mmv = m[0].v * m[0].v, b2v = b[0].v * b[0].v, bb2 = bb[0] * bb[0], bbb = bb[0] * b[0].v;
+#pragma unroll
+    for (int d = 1; d < ND; d++) {
+      mmv = mmv + m[d].v * m[d].v;
+      b2v = b2v + b[d].v * b[d].v;
+      bb2 = bb2 + bb[d] * bb[d];
+      bbb = bbb + bb[d] * b[d].v;
+    }
+    const double pk = (gm - 1) * (e.v - 0.5 * (mmv / rtv) - 0.5 * b2v - bbb);
+    const double cs2 = (gm * (pk + (gm - 1) * (eb - 0.5 * bb2))) / rtv;
+    const double cf = (b2v + bb2 + 2.0 * bbb) / rtv + cs2;
+    const double cs = sqrt(cs2);
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const double btd = bb[d] + b[d].v;
+      const double cfi = 0.5 * (cf + sqrt(cf * cf - 4.0 * cs2 * (btd * btd) / rtv));
+      const double cfd = sqrt(cfi) + fabs(m[d].v / rtv);
+      if (!(cfd <= 1.0e300) || !(cs <= 1.0e300)) bad = 1;
+      const double t = cfd > cs ? cfd : cs;
+      if (t > mx[d]) mx[d] = t;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    double t = mx[d];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(t, off, 64);
+      t = o > t ? o : t;
+    }
+    mx[d] = t;
+  }
+  __shared__ double red[4][3];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+    for (int d = 0; d < ND; d++) red[wid][d] = mx[d];
+  if (bad) atomicOr(flag, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < ND; d++) {
+      double t = red[0][d];
+      for (int q = 1; q < 4; q++) t = red[q][d] > t ? red[q][d] : t;
+      if (t > 0.0) atomicMax(&cmax_bits[d], (unsigned long long)__double_as_longlong(t));
+    }
+  }
+}
+
+// Dual array of n values from value and tangent arrays, and back
+__global__ __launch_bounds__(256) void k_dual_pack(long long n, const double* __restrict__ v,
+                                                   const double* __restrict__ t, Dual* __restrict__ d) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) d[i] = Dual(v[i], t ? t[i] : 0.0);
+}
+__global__ __launch_bounds__(256) void k_dual_unpack(long long n, const Dual* __restrict__ d, double* __restrict__ v,
+                                                     double* __restrict__ t) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  if (v) v[i] = d[i].v;
+  if (t) t[i] = d[i].t;
+}
+
 __global__ void k_zero3(unsigned long long* p) {
   if (threadIdx.x < 3) p[threadIdx.x] = 0ull;
 }
@@ -220,6 +317,24 @@ void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg
       hipLaunchKernelGGL((k_prim<2, double>), dim3((unsigned)nb), dim3(256), 0, st, G, w, bg, (double*)prim,
                          (float*)nullptr, cmax, flag);
   }
+}
+
+void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const double* bg, Dual* prim,
+                    unsigned long long* cmax, int* flag) {
+  long long nb = (G.N + 255) / 256;
+  if (nb > 2048) nb = 2048;
+  if (G.nd == 3)
+    hipLaunchKernelGGL(k_prim_tl<3>, dim3((unsigned)nb), dim3(256), 0, st, G, w, bg, prim, cmax, flag);
+  else
+    hipLaunchKernelGGL(k_prim_tl<2>, dim3((unsigned)nb), dim3(256), 0, st, G, w, bg, prim, cmax, flag);
+}
+
+void launch_dual_pack(hipStream_t st, long long n, const double* v, const double* t, Dual* d) {
+  hipLaunchKernelGGL(k_dual_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, v, t, d);
+}
+
+void launch_dual_unpack(hipStream_t st, long long n, const Dual* d, double* v, double* t) {
+  hipLaunchKernelGGL(k_dual_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, d, v, t);
 }
 
 void launch_zero3(hipStream_t st, unsigned long long* p) {

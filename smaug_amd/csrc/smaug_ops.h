@@ -56,6 +56,57 @@ __device__ inline float rcp_pos(float x) {
   return fmaf(r, e, r);
 }
 
+// Dual number (value v, tangent t) of the tangent-linear step: k_rhs
+// instantiated with it carries d(state) along with the state.  Sums,
+// products and quotients follow the rules of differentiation; comparisons
+// and maxima are not defined on it (the one place they occur, the
+// hyperdiffusion coefficient, is frozen below).  Trivially constructible,
+// so it can fill an LDS array.
+struct Dual {
+  double v, t;
+  Dual() = default;
+  __host__ __device__ constexpr Dual(double a) : v(a), t(0.0) {}
+  __host__ __device__ constexpr Dual(double a, double b) : v(a), t(b) {}
+};
+__host__ __device__ inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.t + b.t); }
+__host__ __device__ inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.t - b.t); }
+__host__ __device__ inline Dual operator-(Dual a) { return Dual(-a.v, -a.t); }
+__host__ __device__ inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.t * b.v + a.v * b.t); }
+__host__ __device__ inline Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.t - q * b.t) / b.v);
+}
+__host__ __device__ inline Dual operator+(Dual a, double b) { return Dual(a.v + b, a.t); }
+__host__ __device__ inline Dual operator+(double a, Dual b) { return Dual(a + b.v, b.t); }
+__host__ __device__ inline Dual operator-(Dual a, double b) { return Dual(a.v - b, a.t); }
+__host__ __device__ inline Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.t); }
+__host__ __device__ inline Dual operator*(Dual a, double b) { return Dual(a.v * b, a.t * b); }
+__host__ __device__ inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.t); }
+__host__ __device__ inline Dual operator/(Dual a, double b) { return Dual(a.v / b, a.t / b); }
+__host__ __device__ inline Dual operator/(double a, Dual b) { return Dual(a) / b; }
+__host__ __device__ inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+__host__ __device__ inline Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+__host__ __device__ inline Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+__device__ inline Dual rcp_pos(Dual x) {
+  const double r = rcp_pos(x.v);
+  return Dual(r, -x.t * r * r);
+}
+
+// own-cell type of the stage kernel for a stencil type: FP64 for the FP64
+// and mixed-precision stencils, Dual for the tangent-linear step
+template <typename SH>
+struct OwnT {
+  using T = double;
+};
+template <>
+struct OwnT<Dual> {
+  using T = Dual;
+};
+// a staged value in the precision sums with the background take
+__device__ inline double widen(float x) { return x; }
+__device__ inline double widen(double x) { return x; }
+__device__ inline Dual widen(Dual x) { return x; }
+
 template <int ND>
 __device__ inline bool in_grid(const Geo& G, const int* x) {
 #pragma unroll
@@ -152,6 +203,21 @@ __device__ inline void nu_win(const SH* __restrict__ qb, int ci, int s, T dxk, T
 #pragma unroll
   for (int o = 0; o < 7; o++) Q[o] = (T)qb[ci + (o - 3) * s];
   nu7<T>(Q, dxk, nur, nul);
+}
+
+// the tangent-linear step holds nu at its value for the state (the
+// derivative is that of the step with the windowed-maximum coefficients
+// frozen, as in the adjoint): nu from the values, tangent 0
+template <>
+__device__ inline void nu_win<Dual, Dual>(const Dual* __restrict__ qb, int ci, int s, Dual dxk, Dual& nur,
+                                          Dual& nul) {
+  double Q[7];
+#pragma unroll
+  for (int o = 0; o < 7; o++) Q[o] = qb[ci + (o - 3) * s].v;
+  double r, l;
+  nu7<double>(Q, dxk.v, r, l);
+  nur = Dual(r);
+  nul = Dual(l);
 }
 
 // fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,

@@ -138,7 +138,7 @@ __device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const
   }
 #pragma unroll
   for (int k = 0; k < NK; k++) {
-    R.v[k][0] = C.with_rho && g[k] >= 0 ? (SH)((double)at(ws, (unsigned)g[k]) + at(bg + (long long)bRHOB() * G.N, (unsigned)g[k]))
+    R.v[k][0] = C.with_rho && g[k] >= 0 ? (SH)(widen(at(ws, (unsigned)g[k])) + at(bg + (long long)bRHOB() * G.N, (unsigned)g[k]))
                                         : SH(0);
 #pragma unroll
     for (int d = 0; d < ND; d++)
@@ -263,7 +263,10 @@ __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __re
 template <int ND, typename SH, bool EDGE>
 __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __restrict__ sm, const int* o) {
   using L = Lay<ND>;
-  using AT = SH;  // arithmetic type of the stencils (the own-cell update and the RK combination are FP64)
+  using AT = SH;  // arithmetic type of the stencils
+  // own-cell type: FP64 (the own-cell update and the RK combination are
+  // FP64 with FP32 stencils too), Dual in the tangent-linear step
+  using OT = typename OwnT<SH>::T;
   constexpr int NP = L::NP;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
   const int tid = threadIdx.x;
@@ -272,7 +275,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
   const bool valid = interior_c<ND>(G, x);
   const long long N = G.N;
   const unsigned c = cix(G, x[0], x[1], x[2]);  // own cell (used by valid threads only)
-  const double* __restrict__ w = A.win;           // own-cell values (FP64)
+  const OT* __restrict__ w = (const OT*)A.win;    // own-cell values
   const SH* __restrict__ ws = (const SH*)A.wsh;    // neighbour values (stencil precision)
   const double* __restrict__ bg = A.bg;
   const SH* __restrict__ pr = (const SH*)A.prim;
@@ -391,7 +394,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     }
   }
 
-  const double rho_c = valid ? at(w, c) : 0.0;
+  const OT rho_c = valid ? at(w, c) : OT(0.0);
   const double rhob_c = valid ? at(bg + (long long)bRHOB() * N, c) : 1.0;
   const AT rt_c = (AT)(rho_c + rhob_c);
 
@@ -714,15 +717,17 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
   }
   // ------------------------------------------------------------ update, source, RK, ghosts
   const double h = A.hfrac * A.dts[0];
-  double out[NP];
+  const OT* __restrict__ base = (const OT*)A.base;
+  OT out[NP];
 #pragma unroll
-  for (int f = 0; f < NP; f++) out[f] = at(A.base + (long long)f * N, c) + h * (double)acc[f];
+  for (int f = 0; f < NP; f++) out[f] = at(base + (long long)f * N, c) + h * (OT)acc[f];
   // divB source terms (divb_parallel, src/divb.cu:165-338, 2D), in FP64 on
   // cells 2 < i,j < n-2: divb from the 4th-order differences over 2 p->dx of
   // b + b_b, zeroed on the layers 3, n-4, n-3 (grad3d's mask with interior
   // transverse indices); m_d += h divb b_d, b_d += h divb m_d/rho_t (a NaN
   // source counts 0); the energy source multiplies wd[bdotv], which the
   // reference never fills, so it is 0
+  if constexpr (sizeof(OT) == 8) {  // FP64 state only (the tangent-linear step excludes divB)
   if (ND == 2 && G.divb && x[0] > 2 && x[1] > 2 && x[0] < G.n[0] - 2 && x[1] < G.n[1] - 2) {
     auto g3 = [&](const double* a, int d) -> double {
       const int xd = x[d], nn = G.n[d];
@@ -740,6 +745,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
       out[fMOM(d)] -= h * (isnan(sm) ? 0.0 : sm);
       out[fB(ND, d)] -= h * (isnan(sb) ? 0.0 : sb);
     }
+  }
   }
   if ((T & SMAUG_T_SOURCE) && G.driver == SMAUG_DRIVER_VERTICAL) {
     const double* dv = G.drv;
@@ -763,9 +769,9 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     }
     // amplitude map: one amplitude per (j,k) column in place of
     // A exp(-dx^2/sxy^2 [- dy^2/sxy^2])
-    const double vz = A.amap ? A.amap[x[1] + (ND == 3 ? (long long)G.n[1] * x[2] : 0)] * ez * A.tdep
-                             : dv[0] * exyz * A.tdep;
-    const double rto = out[fRHO()] + rhob_c;
+    const OT vz = A.amap ? ((const OT*)A.amap)[x[1] + (ND == 3 ? (long long)G.n[1] * x[2] : 0)] * ez * A.tdep
+                         : OT(dv[0] * exyz * A.tdep);
+    const OT rto = out[fRHO()] + rhob_c;
     out[fMOM(0)] += h * vz * rto;
     out[fE(ND)] += h * (vz * vz) * rto / 2.0;
   }
@@ -781,7 +787,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     const double exyz = exx * exy * exz;
     const double vx = (dv[0] * yp / dv[9]) * exyz * A.tdep;
     const double vy = -(dv[0] * xp / dv[8]) * exyz * A.tdep;
-    const double rto = out[fRHO()] + rhob_c;
+    const OT rto = out[fRHO()] + rhob_c;
     out[fMOM(1)] += h * vx * rto;
     out[fMOM(2)] += h * vy * rto;
     out[fE(ND)] += h * (vx * vx + vy * vy) * rto / 2.0;
@@ -800,10 +806,11 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
 #pragma unroll
     for (int f = 0; f < NP; f++) {
       const long long o_f = (long long)f * N;
-      out[f] = out[f] + (at(A.r1 + o_f, c) + 2.0 * at(A.r2 + o_f, c) + at(A.r3 + o_f, c) - 4.0 * at(A.base + o_f, c)) / 3;
+      out[f] = out[f] + (at((const OT*)A.r1 + o_f, c) + 2.0 * at((const OT*)A.r2 + o_f, c) + at((const OT*)A.r3 + o_f, c) -
+                         4.0 * at(base + o_f, c)) / 3;
     }
   }
-  double* __restrict__ wo = A.wout;
+  OT* __restrict__ wo = (OT*)A.wout;
   // the cell itself, then its ghost images: every combination of its images
   // across the directions (a fixed direction has none; with n >= 8 an
   // interior coordinate feeds at most two ghosts per direction), with the
@@ -850,9 +857,11 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
 // faster at 256^3 than four without a spill or six with a 72-byte spill).
 // FP64: three, the number its 52 KB of LDS per workgroup allows (146
 // VGPRs); stating it lets the scheduler spend registers up to that bound
-// (measured 0.8 % faster than a bound of one workgroup).
+// (measured 0.8 % faster than a bound of one workgroup).  Dual (the
+// tangent-linear step, 105 KB of LDS, one workgroup per CU): at most 256
+// registers, the excess spilled to scratch.
 template <int ND, typename SH>
-__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : 3) void k_rhs_tiled(Geo G, RhsArgs A) {
+__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : (sizeof(SH) == 8 ? 3 : 2)) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
   __shared__ SH sm[L::SMEM];
@@ -901,6 +910,15 @@ static void launch_tiled_t(const Geo& G, hipStream_t st, const RhsArgs& A) {
   int nb = ((G.n[0] - 4 + L::TX - 1) / L::TX) * ((G.n[1] - 4 + L::TY - 1) / L::TY);
   if (ND == 3) nb *= (G.n[2] - 4 + L::TZ - 1) / L::TZ;
   hipLaunchKernelGGL((k_rhs_tiled<ND, SH>), dim3(nb), dim3(256), 0, st, G, A);
+}
+
+// the tangent-linear stage: every state array of A (win, wsh, prim, base,
+// r1..r3, wout and the driver map) holds Dual values
+void launch_rhs_tl(const Geo& G, hipStream_t st, const RhsArgs& A) {
+  if (G.nd == 3)
+    launch_tiled_t<3, Dual>(G, st, A);
+  else
+    launch_tiled_t<2, Dual>(G, st, A);
 }
 
 void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A) {

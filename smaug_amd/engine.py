@@ -25,6 +25,7 @@ EXPORTS = [
     "smaug_ini_read", "smaug_ini_write", "smaug_out_write", "smaug_write_out",
     "smaug_set_driver_map", "smaug_set_stream", "smaug_set_clock", "smaug_save_pert", "smaug_load_pert",
     "smaug_tape_doubles", "smaug_step_taped", "smaug_adjoint_step", "smaug_cfl_noalloc",
+    "smaug_tl_begin", "smaug_tl_step", "smaug_tl_get", "smaug_tl_end",
 ]
 
 
@@ -69,6 +70,10 @@ def load() -> ctypes.CDLL:
     L.smaug_tape_doubles.argtypes = [vp]
     L.smaug_step_taped.argtypes = [vp, vp]
     L.smaug_adjoint_step.argtypes = [vp, vp, vp, vp, vp]
+    L.smaug_tl_begin.argtypes = [vp, dp, dp]
+    L.smaug_tl_step.argtypes = [vp, ctypes.c_int]
+    L.smaug_tl_get.argtypes = [vp, dp, dp]
+    L.smaug_tl_end.argtypes = [vp]
     L.smaug_cfl_noalloc.argtypes = [vp, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
@@ -206,6 +211,34 @@ class Engine:
         _chk(load().smaug_adjoint_step(self._h, ctypes.c_void_p(tape_ptr), ctypes.c_void_p(wnext_ptr),
                                        ctypes.c_void_p(lam_ptr), ctypes.c_void_p(gmap_ptr or None)),
              "smaug_adjoint_step")
+
+    # ---- tangent-linear run (forward-mode derivative; host arrays)
+    def tl_begin(self, dw0: np.ndarray, damap: Optional[np.ndarray] = None):
+        """Starts a tangent-linear run from the current state in the direction
+        dw0 (np x N evolved fields) and, with a driver map set, damap (one
+        value per (j, k) column)."""
+        a = np.ascontiguousarray(dw0, dtype=np.float64).reshape(-1)
+        if a.size != self.npert * self.N:
+            raise ValueError("dw0 needs np*N values")
+        m = None
+        if damap is not None:
+            m = np.ascontiguousarray(damap, dtype=np.float64).reshape(-1)
+            if m.size != self.N // self.rp.n[0]:
+                raise ValueError("damap needs one value per (j, k) column")
+        _chk(load().smaug_tl_begin(self._h, _dp(a), None if m is None else _dp(m)), "smaug_tl_begin")
+
+    def tl_step(self, n: int = 1):
+        _chk(load().smaug_tl_step(self._h, int(n)), "smaug_tl_step")
+
+    def tl_get(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(state, tangent) of the run, each np x N."""
+        w = np.empty((self.npert, self.N))
+        dw = np.empty((self.npert, self.N))
+        _chk(load().smaug_tl_get(self._h, _dp(w), _dp(dw)), "smaug_tl_get")
+        return w, dw
+
+    def tl_end(self):
+        _chk(load().smaug_tl_end(self._h), "smaug_tl_end")
 
     def time_kernels(self, nsteps: int) -> Tuple[float, float, float]:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

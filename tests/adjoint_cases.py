@@ -86,3 +86,36 @@ class ReplayFD:
 
     def close(self):
         self.o.close()
+
+
+def line_loss_grad_np(rp, w: np.ndarray, rhob: np.ndarray, i_line: int) -> np.ndarray:
+    """d line_loss / d w for the evolved fields w (np x N), rho_b given (N)."""
+    nd = rp.ndim
+    n = rp.n
+    shape = tuple(n[:nd][::-1])
+    v = w.reshape((w.shape[0],) + shape)
+    g = np.zeros_like(v)
+    inner = tuple(slice(2, n[d] - 2) for d in range(nd - 1, 0, -1)) + (i_line,)
+    rt = v[0][inner] + rhob.reshape(shape)[inner]
+    vz = v[1][inner] / rt
+    g[1][inner] = 2 * vz / rt
+    g[0][inner] = -2 * vz * vz / rt
+    return g.reshape(w.shape)
+
+
+def tl_directional(eng, rp, rhob, delta, damap, nsteps: int, i_line: int):
+    """(sum_n line_loss(w_n), sum_n d line_loss(w_n)) along the engine's
+    tangent-linear run from its current state in the direction (delta,
+    damap)."""
+    eng.tl_begin(delta, damap)
+    loss = dloss = 0.0
+    for _ in range(nsteps):
+        eng.tl_step(1)
+        w, dw = eng.tl_get()
+        full = np.zeros((3 + 2 * rp.ndim + 1, w.shape[1]))
+        full[: w.shape[0]] = w
+        full[3 + 2 * rp.ndim] = rhob  # line_loss_np reads rho_b from the background rows
+        loss += line_loss_np(rp, full, i_line)
+        dloss += float((line_loss_grad_np(rp, w, rhob, i_line) * dw).sum())
+    eng.tl_end()
+    return loss, dloss

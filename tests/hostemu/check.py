@@ -8,6 +8,8 @@ and kernels the GPU build compiles.
   python -m tests.hostemu.check mixed <case>           mixed precision vs oracle
   python -m tests.hostemu.check adj  <case> <terms>    adjoint vs oracle FD
   python -m tests.hostemu.check autograd <rkon>        smaug_amd.adjoint path
+  python -m tests.hostemu.check tl   <case> <terms>    tangent-linear run vs oracle FD
+                                                       and vs the adjoint (dot products)
 
 case: s2r (2D stratified RK4), s2e (2D Euler), s3r (3D RK4), s3e (3D Euler,
 mixed only), ot (2D periodic).
@@ -25,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 from smaug_amd import params as P  # noqa: E402
 from smaug_amd.engine import Engine  # noqa: E402
-from tests.adjoint_cases import ReplayFD, interior_mask  # noqa: E402
+from tests.adjoint_cases import ReplayFD, interior_mask, tl_directional  # noqa: E402
 from tests.cases import compare, ot_case, strat_case  # noqa: E402
 import oracle.pyoracle as oracle  # noqa: E402
 
@@ -193,6 +195,45 @@ def check_adj(case: str, terms: int):
     if amap is not None:
         da = rng.standard_normal(amap.size) * np.abs(amap).mean()
         out["amap"] = [float((ga * da).sum()), fd.dir_amap(da, 1e-6)]
+    fd.close()
+    return out
+
+
+def check_tl(case: str, terms: int):
+    """Directional derivatives of the line loss along the tangent-linear run
+    (smaug_tl_*) against the replayed-coefficient differences of the oracle,
+    one direction per evolved field and one in the driver map, and the same
+    directions through the adjoint (<adjoint gradient, direction> equals the
+    tangent-linear derivative up to roundoff: the two are transposes)."""
+    rp, w, pos, amap, il, nsteps = make_case(case, terms)
+    gw, ga = engine_adjoint(rp, w, pos, amap, nsteps, il)
+    fd = ReplayFD(oracle, rp, w, pos, amap, nsteps, il)
+    e = Engine(rp, w, pos)
+    if amap is not None:
+        e.set_driver_map(amap)
+    rhob = w[3 + 2 * rp.ndim]
+    rng = np.random.default_rng(7)
+    k = P.npert(rp.ndim)
+    inner = interior_mask(rp)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1)) + 1e-30
+    out = {"fields": [], "adjoint": []}
+    loss = None
+    s0 = e.state()
+    for f in range(k):
+        delta = np.zeros((k, rp.ncells))
+        delta[f] = rng.standard_normal(rp.ncells) * scale[f] * inner
+        loss, tl = tl_directional(e, rp, rhob, delta, None, nsteps, il)
+        out["fields"].append([tl, fd.dir_w0(delta, 1e-6)])
+        out["adjoint"].append([tl, float((gw * delta).sum())])
+    out["loss"] = [loss, fd.base]
+    if amap is not None:
+        da = rng.standard_normal(amap.size) * np.abs(amap).mean()
+        _, tl = tl_directional(e, rp, rhob, np.zeros((k, rp.ncells)), da, nsteps, il)
+        out["amap"] = [tl, fd.dir_amap(da, 1e-6)]
+        out["amap_adjoint"] = [tl, float((ga * da).sum())]
+    # the run leaves the engine's own state alone
+    out["state_untouched"] = bool(np.array_equal(e.state(), s0))
+    e.close()
     fd.close()
     return out
 
@@ -367,6 +408,8 @@ def main(argv):
         r = check_mixed(argv[1])
     elif what == "adj":
         r = check_adj(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
+    elif what == "tl":
+        r = check_tl(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
     else:
         r = check_autograd(int(argv[1]))
     print(json.dumps(r), flush=True)

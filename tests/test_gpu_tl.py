@@ -1,0 +1,152 @@
+"""Tangent-linear run (smaug_tl_*) on the MI355X: the stage kernel
+instantiated on (value, tangent) pairs, with the hyperdiffusion coefficients,
+the wave-speed maxima and dt held at their values along the trajectory.
+
+Checked against the oracle with its hyperdiffusion coefficients replayed
+from the unperturbed run (the derivative of the step with those
+coefficients frozen), by Richardson-extrapolated central differences at
+h = 1e-6 relative: the directional derivative of the summed line loss
+sum_n sum_line (m_0 / rho_t)^2 agrees to 1e-6 relative, per evolved field.
+The values of the run reproduce the oracle's loss (1e-12) and the FP64
+forward step's state (1e-13 per variable), and the run leaves the engine's
+own state untouched.  The driver amplitude map stays off here: it belongs
+to the differentiable entry points gated off on the GPU (SMAUG_ADJOINT), so
+its direction is checked in the host build (tests/test_hostemu.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from smaug_amd import params as P
+from tests.adjoint_cases import ReplayFD, interior_mask, tl_directional
+from tests.cases import compare, ot_case, strat_case
+
+oracle = pytest.importorskip("oracle.pyoracle")
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+EPS = 1e-6
+
+
+def tl_check(rp, w, pos, nsteps, i_line, per_field=True, seed=7):
+    from smaug_amd.engine import Engine
+
+    print(f"[tl] {rp.ndim}D {rp.n} rkon={rp.rkon} terms={rp.terms:#x} driver={rp.driver} steps={nsteps}", flush=True)
+    e = Engine(rp, w, pos)
+    s0 = e.state()
+    fd = ReplayFD(oracle, rp, w, pos, None, nsteps, i_line)
+    try:
+        k = P.npert(rp.ndim)
+        rhob = w[3 + 2 * rp.ndim]
+        inner = interior_mask(rp)
+        rng = np.random.default_rng(seed)
+        scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1)) + 1e-30
+        dirs = []
+        if per_field:
+            for f in range(k):
+                d = np.zeros((k, rp.ncells))
+                d[f] = rng.standard_normal(rp.ncells) * scale[f] * inner
+                dirs.append(d)
+        else:
+            dirs.append(rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :])
+        for i, d in enumerate(dirs):
+            loss, got = tl_directional(e, rp, rhob, d, None, nsteps, i_line)
+            want = fd.dir_w0(d, EPS)
+            print(f"  w0 direction {i}: tangent-linear {got:.12e} fd {want:.12e}", flush=True)
+            assert abs(loss - fd.base) <= 1e-12 * abs(fd.base), (loss, fd.base)
+            if want == 0.0:
+                assert abs(got) <= 1e-12, (i, got)
+            else:
+                assert abs(got - want) <= TOL * abs(want), (i, got, want)
+        # the run's values against the FP64 forward step of the same engine
+        e.tl_begin(np.zeros((k, rp.ncells)))
+        e.tl_step(nsteps)
+        wv, dw = e.tl_get()
+        e.tl_end()
+        assert np.array_equal(e.state(), s0)
+        assert not np.any(dw)
+        e.step(nsteps)
+        ref = e.state()
+        full = ref.copy()
+        full[:k] = wv
+        assert max(compare(rp, full, ref)) <= 1e-13
+    finally:
+        fd.close()
+        e.close()
+
+
+def strat_tl_case(ni, ndim, rkon, terms=P.TERMS_ALL):
+    rp, w, pos = strat_case(ni=ni, ndim=ndim, rkon=rkon)
+    rp.terms = terms
+    return rp, w, pos
+
+
+@pytest.mark.parametrize("rkon", [1, 0])
+def test_tl_strat2d(rkon):
+    rp, w, pos = strat_tl_case((16, 12), 2, rkon)
+    tl_check(rp, w, pos, nsteps=4, i_line=7)
+
+
+def test_tl_strat3d_rk4():
+    rp, w, pos = strat_tl_case((12, 8, 8), 3, 1)
+    tl_check(rp, w, pos, nsteps=3, i_line=6)
+
+
+@pytest.mark.parametrize("group", ["flux", "eterm", "grav", "hrho", "he", "hmom", "hb"])
+def test_tl_term_groups_2d(group):
+    t = {"flux": P.T_FLUX, "eterm": P.T_ETERM, "grav": P.T_GRAV, "hrho": P.T_HRHO, "he": P.T_HE,
+         "hmom": P.T_HMOM, "hb": P.T_HB}[group]
+    rp, w, pos = strat_tl_case((16, 12), 2, 1, terms=t | P.T_SOURCE)
+    tl_check(rp, w, pos, nsteps=3, i_line=7)
+
+
+@pytest.mark.parametrize("group", ["flux", "hmom", "hb"])
+def test_tl_term_groups_3d(group):
+    t = {"flux": P.T_FLUX | P.T_ETERM | P.T_GRAV, "hmom": P.T_HMOM | P.T_HRHO | P.T_HE, "hb": P.T_HB}[group]
+    rp, w, pos = strat_tl_case((12, 8, 8), 3, 1, terms=t | P.T_SOURCE)
+    tl_check(rp, w, pos, nsteps=2, i_line=6)
+
+
+def test_tl_orszag_tang_periodic():
+    """Periodic boundaries with the flux and background terms.  With
+    hyperdiffusion on, the Orszag-Tang derivative with frozen coefficients is
+    itself set by roundoff: the smooth initial data leave third differences
+    at roundoff level, so nu along a trajectory started one ulp away differs
+    at O(1) on some interfaces and the oracle's own replayed derivative moves
+    by up to 4 % (16^2, 3 steps; 8e-13 on the stratified case).  That case is
+    checked in the host build, whose arithmetic reproduces the oracle's nu."""
+    rp, w, pos = ot_case(ni=16, rkon=1)
+    rp.moddton = 0
+    rp.dt = 2e-3
+    rp.terms = P.TERMS_ALL & ~(P.T_HRHO | P.T_HE | P.T_HMOM | P.T_HB)
+    tl_check(rp, w, pos, nsteps=3, i_line=9)
+
+
+def test_tl_inner_tiles_3d():
+    """A 3D grid with tiles away from every face (the stage kernel's inner
+    body) as well as edge tiles: one direction over all fields."""
+    rp, w, pos = strat_tl_case((30, 17, 37), 3, 1)
+    tl_check(rp, w, pos, nsteps=2, i_line=9, per_field=False)
+
+
+@pytest.mark.parametrize("case", ["tor", "spi"])
+def test_tl_reference_drivers(case):
+    from tests.test_drivers import spicule_case, torsional_case
+
+    if case == "tor":
+        rp, w, pos = torsional_case(rkon=1, terms=P.TERMS_ALL)
+        tl_check(rp, w, pos, nsteps=2, i_line=6)
+    else:
+        rp, w, pos = spicule_case(ndim=2, rkon=1, terms=P.TERMS_ALL)
+        tl_check(rp, w, pos, nsteps=3, i_line=7)
+
+
+@pytest.mark.parametrize("key,rkon", [("2d_fixed_asymm", 0), ("3d_cont_symm_fixed", 1)])
+def test_tl_boundary_types(key, rkon):
+    from tests.test_boundaries import CASES, bc_case
+
+    nd, types, ft = CASES[key]
+    rp, w, pos = bc_case(nd, rkon, types, ft)
+    tl_check(rp, w, pos, nsteps=3 if nd == 2 else 2, i_line=7 if nd == 2 else 6)

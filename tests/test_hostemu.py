@@ -11,7 +11,9 @@ thread as a fiber and guard pages around every buffer, so
     hyperdiffusion coefficients replayed, to 1e-6 relative, per evolved
     field and for the driver amplitude map, for every term group;
   * the torch.autograd.Function path (smaug_amd.adjoint) gives the same
-    gradient for every checkpoint interval and matches the differences.
+    gradient for every checkpoint interval and matches the differences;
+  * the tangent-linear run (smaug_tl_*) matches the differences and is the
+    transpose of the adjoint (dot products equal to roundoff).
 """
 from __future__ import annotations
 
@@ -87,6 +89,33 @@ def test_adjoint_matches_replayed_differences(hostlib, case, align):
         assert close(pair), (f, pair)
     if "amap" in r:
         assert close(r["amap"]), r["amap"]
+
+
+def transposed(pair, tol=1e-10):
+    """The tangent-linear derivative and the adjoint's dot product in one
+    direction: the same linear map read forwards and transposed, so equal up
+    to the roundoff of the two sums."""
+    got, want = pair
+    return abs(got - want) <= tol * abs(want)
+
+
+@pytest.mark.parametrize("case", ["s2r", "s2e", "s3r", "ot", "tor", "bc:2d_fixed_asymm:0"])
+def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
+    """The tangent-linear run (smaug_tl_*): its values reproduce the oracle's
+    loss, its directional derivatives of the line loss match the replayed
+    differences of the oracle (1e-6) and equal the adjoint's dot products in
+    the same directions (1e-10), per evolved field and for the driver map; the
+    engine's own state is left as it was."""
+    r = run(hostlib, "tl", case)
+    assert abs(r["loss"][0] - r["loss"][1]) <= 1e-12 * abs(r["loss"][1]), r["loss"]
+    for f, pair in enumerate(r["fields"]):
+        assert close(pair), (f, pair)
+    for f, pair in enumerate(r["adjoint"]):
+        assert transposed(pair), (f, pair)
+    if "amap" in r:
+        assert close(r["amap"]), r["amap"]
+        assert transposed(r["amap_adjoint"]), r["amap_adjoint"]
+    assert r["state_untouched"]
 
 
 BC_CASES = ["bc:2d_symm_cont:1", "bc:2d_fixed_asymm:0", "bc:3d_cont_symm_fixed:1", "bc:3d_asymm_fixed_cont:1"]

@@ -55,6 +55,12 @@ def test_k_rhs_occupancy_and_scratch(tmp_path):
     for k, v in k32.items():
         assert v["Occupancy"] >= 5, (k, v)
         assert v["ScratchSize"] <= 16, (k, v)
+    # the tangent-linear (Dual) instantiations: within the 256 architected
+    # registers, no accumulation registers, no scratch
+    kd = {k: v for k, v in report.items() if "k_rhs_tiled" in k and "Dual" in k}
+    assert len(kd) == 2, report
+    for k, v in kd.items():
+        assert v["VGPRs"] <= 256 and v.get("AGPRs", 0) == 0 and v["ScratchSize"] == 0, (k, v)
     assert ARCH == "gfx950"
 
 
