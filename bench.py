@@ -93,6 +93,7 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=256, help="interior cells per dim")
     ap.add_argument("--kernel-steps", type=int, default=3, help="steps timed per kernel with events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tl-steps", type=int, default=3, help="tangent-linear steps timed (0 skips)")
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--mixed-check-steps", type=int, default=2000,
@@ -146,6 +147,21 @@ def main(argv=None):
     ms_step = wall / args.steps * 1e3
     state64 = eng.state()  # before the event-timed steps, to compare with the mixed run
     ms_prim, ms_rhs, ms_evstep = eng.time_kernels(args.kernel_steps)
+    # tangent-linear step (forward-mode derivative, dual-number stage kernel)
+    # from the same state: one untimed step, then timed steps
+    tl = None
+    if args.tl_steps > 0:
+        eng.tl_begin(np.zeros((2 + 2 * rp.ndim, rp.ncells)))
+        eng.tl_step(1)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.tl_step(args.tl_steps)
+        eng.sync()
+        ms_tl = (time.perf_counter() - t0) / args.tl_steps * 1e3
+        eng.tl_end()
+        print(f"[bench] tangent-linear: timed {args.tl_steps} steps, {ms_tl:.3f} ms/step", file=sys.stderr, flush=True)
+        tl = {"ms_per_step": ms_tl, "over_forward": ms_tl / ms_step, "steps": args.tl_steps,
+              "note": "FP64 tangent-linear step (values and tangents, frozen nu, wave-speed maxima and dt)"}
     eng.close()
 
     # mixed precision: FP32 stencil shadows, FP64 state/CFL/combine, same
@@ -253,6 +269,7 @@ def main(argv=None):
             "scheme": "rk4",
         },
         "mixed": mixed,
+        "tangent_linear": tl,
         "adjoint": {
             "adjoint_over_forward": None,
             "status": "not measured: the adjoint entry points are gated off on the GPU (SMAUG_ADJOINT) "
