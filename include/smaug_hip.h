@@ -278,13 +278,15 @@ int smaug_adjoint_step(smaug_t* s, const double* tape_device, const double* wnex
  * clock do not change.  Host arrays.  divbon = 1 is rejected.
  *   smaug_tl_begin -- starts a run from the current state in the direction
  *                     dw0 (np*N doubles, field-major; its ghost cells are
- *                     re-filled with the boundary maps) and, when a driver
+ *                     re-filled with the boundary maps), dbg (the static
+ *                     background fields, nb*N doubles: e_b, rho_b, b_b per
+ *                     cell, ghosts included; NULL = 0) and, when a driver
  *                     amplitude map is set, damap (n1*n2 values, NULL = 0).
  *   smaug_tl_step  -- advances the run by nsteps steps.
  *   smaug_tl_get   -- copies the run's state (w, np*N) and its tangent (dw,
  *                     np*N); either may be NULL.
  *   smaug_tl_end   -- frees the run (smaug_destroy does too). */
-int smaug_tl_begin(smaug_t* s, const double* dw0_host, const double* damap_host);
+int smaug_tl_begin(smaug_t* s, const double* dw0_host, const double* dbg_host, const double* damap_host);
 int smaug_tl_step(smaug_t* s, int nsteps);
 int smaug_tl_get(smaug_t* s, double* w_host, double* dw_host);
 int smaug_tl_end(smaug_t* s);

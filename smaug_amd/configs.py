@@ -163,17 +163,18 @@ def spicule1_driver(rp: RunParams, AA: float = 200.0, tdep: float = 1.0, centre=
 
 
 def stratified(rp: RunParams, b_tesla: float = 0.01, perturb: Optional[int] = None,
-               amp: float = 1.0e-4) -> Tuple[np.ndarray, np.ndarray]:
+               amp: float = 1.0e-4, t_photo: float = 6000.0, z_tr: float = 2.0e6) -> Tuple[np.ndarray, np.ndarray]:
     """Hydrostatic background with a uniform vertical field; perturbation
     zero, or seeded noise (seed `perturb`) smoothed by three passes of a
     3-point box filter: drho/rho_b ~ N(0, amp), de ~ N(0, amp) p/(gamma-1),
-    v ~ N(0, amp c_s), db ~ N(0, amp |b_b|)."""
+    v ~ N(0, amp c_s), db ~ N(0, amp |b_b|).  t_photo and z_tr are the
+    photospheric temperature and the transition height of T(z)."""
     coords = _mesh(rp)
     n0 = rp.n[0]
     gam = rp.gamma
     # the background depends on dim 0 (height) only: build it on the axis
     zax = coords[0][:n0]
-    p_ax, rho_ax = hydrostatic(zax, grav=-rp.g[0] if rp.g[0] != 0 else 274.0)
+    p_ax, rho_ax = hydrostatic(zax, grav=-rp.g[0] if rp.g[0] != 0 else 274.0, t_lo=t_photo, z_tr=z_tr)
     reps = rp.ncells // n0
     p = np.tile(p_ax, reps)
     rho_b = np.tile(rho_ax, reps)
@@ -215,3 +216,34 @@ def stratified(rp: RunParams, b_tesla: float = 0.01, perturb: Optional[int] = No
             w[2 + nd + d] = smooth(rng.normal(0.0, amp, ncell)) * bz
     pos = np.stack(coords)
     return w, pos
+
+
+STATE_CONTROLS = ("t_photo", "z_tr")
+
+
+def stratified_tangent(rp: RunParams, control: str, rel: float = 1.0e-4, **kw) -> np.ndarray:
+    """d(state)/d(control) of the stratified initial state (nvar x N: the
+    perturbation, which the seeded noise scales with the background, and the
+    background fields), for control 't_photo' or 'z_tr' at the values in kw
+    (defaults 6000 K, 2 Mm).  Richardson-extrapolated central differences of
+    the closed-form builder at steps rel and 2 rel of the control: truncation
+    O(rel^4), roundoff ~1e-16/rel, ~1e-11 relative at the default.  The
+    background direction for smaug_tl_begin (dbg) is rows np.., the evolved
+    direction (dw0) rows ..np."""
+    if control not in STATE_CONTROLS:
+        raise ValueError(f"control must be one of {STATE_CONTROLS}")
+    base = dict(t_photo=6000.0, z_tr=2.0e6)
+    base.update(kw)
+    c0 = base[control]
+
+    def at(h):
+        args = dict(base)
+        args[control] = c0 + h
+        return stratified(rp, **args)[0]
+
+    h = rel * abs(c0)
+
+    def central(hh):
+        return (at(hh) - at(-hh)) / (2.0 * hh)
+
+    return (4.0 * central(h) - central(2.0 * h)) / 3.0

@@ -313,7 +313,7 @@ void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
 // (value, tangent) arrays, with nu and the wave-speed maxima frozen
 struct Dual;
 void launch_rhs_tl(const Geo& G, hipStream_t st, const RhsArgs& A);
-void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const double* bg, Dual* prim,
+void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const Dual* bg, Dual* prim,
                     unsigned long long* cmax, int* flag);
 // d[i] = (v[i], t[i]) (t null: tangent 0), and back (either output may be null)
 void launch_dual_pack(hipStream_t st, long long n, const double* v, const double* t, Dual* d);

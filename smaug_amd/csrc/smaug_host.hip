@@ -101,6 +101,7 @@ struct smaug_ctx {
   // and driver map, its own maxima slots, (dt, t), clock and host staging
   Dual* tlU[4] = {nullptr, nullptr, nullptr, nullptr};
   Dual* tlprim = nullptr;
+  Dual* tlbg = nullptr;  // background with its tangent
   Dual* tlamap = nullptr;
   unsigned long long* tlcmax = nullptr;  // 2 slots of 4
   double* tldts = nullptr;
@@ -116,11 +117,12 @@ static void free_tl(smaug_t* s) {
     u = nullptr;
   }
   hipFree(s->tlprim);
+  hipFree(s->tlbg);
   hipFree(s->tlamap);
   hipFree(s->tlcmax);
   hipFree(s->tldts);
   hipFree(s->tltmp);
-  s->tlprim = s->tlamap = nullptr;
+  s->tlprim = s->tlbg = s->tlamap = nullptr;
   s->tlcmax = nullptr;
   s->tldts = s->tltmp = nullptr;
   s->tl = false;
@@ -907,7 +909,7 @@ extern "C" int smaug_adjoint_step(smaug_t* s, const double* tape, const double* 
 // values (the derivative the adjoint takes).  Own buffers, own maxima slots
 // and clock: the context's state and clock are left as they are.
 
-extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* damap) {
+extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* dbg, const double* damap) {
   if (!s || !dw0) return fail(SMAUG_E_ARG, "null argument");
   if (s->G.divb) return fail(SMAUG_E_UNSUPPORTED, "the tangent-linear step does not include the divB source terms");
   if (damap && !s->amap) return fail(SMAUG_E_ARG, "a driver-map direction needs a driver amplitude map (smaug_set_driver_map)");
@@ -921,10 +923,12 @@ extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* damap
   bool ok = true;
   for (int b = 0; b < nbuf; b++) ok = ok && hipMalloc(&s->tlU[b], sizeof(Dual) * (size_t)F) == hipSuccess;
   ok = ok && hipMalloc(&s->tlprim, sizeof(Dual) * (size_t)(s->npr * s->N)) == hipSuccess;
+  ok = ok && hipMalloc(&s->tlbg, sizeof(Dual) * (size_t)(s->nb * s->N)) == hipSuccess;
   if (s->amap) ok = ok && hipMalloc(&s->tlamap, sizeof(Dual) * (size_t)ncol) == hipSuccess;
   ok = ok && hipMalloc(&s->tlcmax, sizeof(unsigned long long) * 8) == hipSuccess;
   ok = ok && hipMalloc(&s->tldts, sizeof(double) * 2) == hipSuccess;
-  ok = ok && hipMalloc(&s->tltmp, sizeof(double) * (size_t)(2 * F)) == hipSuccess;
+  const long long B = (long long)s->nb * s->N;
+  ok = ok && hipMalloc(&s->tltmp, sizeof(double) * (size_t)(2 * F > B ? 2 * F : B)) == hipSuccess;
   if (!ok) {
     free_tl(s);
     return fail(SMAUG_E_HIP, "tangent-linear buffers: allocation failed");
@@ -939,6 +943,9 @@ extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* damap
   launch_dual_pack(st, F, s->U[s->cur], s->tltmp, s->tlU[0]);
   for (int b = 1; b < nbuf; b++)  // fixed ghosts of every stage buffer hold the initial values
     HIPCHK(hipMemcpyAsync(s->tlU[b], s->tlU[0], sizeof(Dual) * (size_t)F, hipMemcpyDeviceToDevice, st));
+  // the background and its direction (every cell, ghosts included, as given)
+  if (dbg) HIPCHK(hipMemcpyAsync(s->tltmp, dbg, sizeof(double) * (size_t)B, hipMemcpyHostToDevice, st));
+  launch_dual_pack(st, B, s->bg, dbg ? s->tltmp : nullptr, s->tlbg);
   if (s->tlamap) {
     double* dm = s->tltmp + F;
     if (damap)
@@ -962,13 +969,13 @@ static void tl_stage(smaug_t* s, const Geo& Gt, Dual* win, const Dual* base, Dua
                      bool first, const Dual* r1, const Dual* r2, const Dual* r3) {
   hipStream_t st = s->stream;
   const int slot = s->tlslot;
-  launch_prim_tl(Gt, st, win, s->bg, s->tlprim, s->tlcmax + 4 * slot, s->flag);
+  launch_prim_tl(Gt, st, win, s->tlbg, s->tlprim, s->tlcmax + 4 * slot, s->flag);
   if (first)
     launch_dt(st, s->tlcmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->tldts);
   RhsArgs A;
   A.win = (const double*)win;
   A.wsh = win;
-  A.bg = s->bg;
+  A.bg = (const double*)s->tlbg;
   A.prim = s->tlprim;
   A.delx = s->delx;
   A.rdelx = s->rdelx;

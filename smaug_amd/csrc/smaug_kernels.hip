@@ -132,12 +132,12 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
   }
 }
 
-// k_prim of the tangent-linear step: w holds Dual values (value and
-// tangent of every evolved field); v and q_e are formed in Dual arithmetic
+// k_prim of the tangent-linear step: w and bg hold Dual values (value and
+// tangent of every evolved and background field); v and q_e are formed in Dual arithmetic
 // (the same expressions as k_prim), the wave speeds and their maxima from
 // the values alone (frozen, as the hyperdiffusion coefficients they scale)
 template <int ND>
-__global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__ w, const double* __restrict__ bg,
+__global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__ w, const Dual* __restrict__ bg,
                                                  Dual* __restrict__ prim, unsigned long long* __restrict__ cmax_bits,
                                                  int* __restrict__ flag) {
   const long long N = G.N;
@@ -146,9 +146,8 @@ __global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__
   const long long stride = (long long)gridDim.x * 256;
   for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += stride) {
     const Dual rho = w[c], e = w[(long long)fE(ND) * N + c];
-    const double rhob = bg[(long long)bRHOB() * N + c], eb = bg[(long long)bEB() * N + c];
-    Dual m[ND], b[ND];
-    double bb[ND];
+    const Dual rhob = bg[(long long)bRHOB() * N + c], eb = bg[(long long)bEB() * N + c];
+    Dual m[ND], b[ND], bb[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       m[d] = w[(long long)fMOM(d) * N + c];
@@ -167,21 +166,21 @@ __global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__
     prim[(long long)pQE(ND) * N + c] = e - 0.5 * (b2 + mm / rt);
     // wave speeds of the values (k_prim's expressions)
     const double gm = G.gamma, rtv = rt.v;
-    double mmv = m[0].v * m[0].v, b2v = b[0].v * b[0].v, bb2 = bb[0] * bb[0], bbb = bb[0] * b[0].v;
+    double mmv = m[0].v * m[0].v, b2v = b[0].v * b[0].v, bb2 = bb[0].v * bb[0].v, bbb = bb[0].v * b[0].v;
 #pragma unroll
     for (int d = 1; d < ND; d++) {
       mmv = mmv + m[d].v * m[d].v;
       b2v = b2v + b[d].v * b[d].v;
-      bb2 = bb2 + bb[d] * bb[d];
-      bbb = bbb + bb[d] * b[d].v;
+      bb2 = bb2 + bb[d].v * bb[d].v;
+      bbb = bbb + bb[d].v * b[d].v;
     }
     const double pk = (gm - 1) * (e.v - 0.5 * (mmv / rtv) - 0.5 * b2v - bbb);
-    const double cs2 = (gm * (pk + (gm - 1) * (eb - 0.5 * bb2))) / rtv;
+    const double cs2 = (gm * (pk + (gm - 1) * (eb.v - 0.5 * bb2))) / rtv;
     const double cf = (b2v + bb2 + 2.0 * bbb) / rtv + cs2;
     const double cs = sqrt(cs2);
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      const double btd = bb[d] + b[d].v;
+      const double btd = bb[d].v + b[d].v;
       const double cfi = 0.5 * (cf + sqrt(cf * cf - 4.0 * cs2 * (btd * btd) / rtv));
       const double cfd = sqrt(cfi) + fabs(m[d].v / rtv);
       if (!(cfd <= 1.0e300) || !(cs <= 1.0e300)) bad = 1;
@@ -319,7 +318,7 @@ void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg
   }
 }
 
-void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const double* bg, Dual* prim,
+void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const Dual* bg, Dual* prim,
                     unsigned long long* cmax, int* flag) {
   long long nb = (G.N + 255) / 256;
   if (nb > 2048) nb = 2048;

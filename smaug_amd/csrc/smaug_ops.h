@@ -228,8 +228,8 @@ struct CellState {
   T m[ND], b[ND], bb[ND], rho, rhob, e, eb;
 };
 
-template <int ND, typename SH, typename T = double>
-__device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const double* __restrict__ bg, unsigned cc,
+template <int ND, typename SH, typename T = double, typename BT = double>
+__device__ inline void load_cell(const Geo& G, const SH* __restrict__ w, const BT* __restrict__ bg, unsigned cc,
                                  CellState<ND, T>& S) {
   const long long N = G.N;
 #pragma unroll

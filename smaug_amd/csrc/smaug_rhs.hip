@@ -123,7 +123,7 @@ struct CellLoads {
 };
 
 template <int ND, typename SH, bool EDGE>
-__device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const double* __restrict__ bg,
+__device__ inline void cells_load(const Geo& G, const SH* __restrict__ ws, const typename OwnT<SH>::T* __restrict__ bg,
                                   const CellBox<ND, SH>& C, const int* o, CellLoads<ND, SH>& R) {
   using L = Lay<ND>;
   constexpr int NK = CellLoads<ND, SH>::NK;
@@ -202,7 +202,7 @@ __device__ inline void stage_q_edge(const Geo& G, const SH* __restrict__ qa, SH*
 // caller's tile lies at least five cells inside the grid (the box is inside)
 template <int ND, typename SH, bool EDGE>
 __device__ inline void stage_q(const Geo& G, const SH* __restrict__ qa, SH* __restrict__ qb, const int* o,
-                               const SH* __restrict__ ws = nullptr, const double* __restrict__ bg = nullptr,
+                               const SH* __restrict__ ws = nullptr, const typename OwnT<SH>::T* __restrict__ bg = nullptr,
                                CellBox<ND, SH> C = CellBox<ND, SH>{nullptr, false, 0}, const SH* __restrict__ qa2 = nullptr,
                                SH* __restrict__ qb2 = nullptr) {
   using L = Lay<ND>;
@@ -277,7 +277,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
   const unsigned c = cix(G, x[0], x[1], x[2]);  // own cell (used by valid threads only)
   const OT* __restrict__ w = (const OT*)A.win;    // own-cell values
   const SH* __restrict__ ws = (const SH*)A.wsh;    // neighbour values (stencil precision)
-  const double* __restrict__ bg = A.bg;
+  const OT* __restrict__ bg = (const OT*)A.bg;  // background (Dual in the tangent-linear step)
   const SH* __restrict__ pr = (const SH*)A.prim;
   const unsigned T = G.terms;
   AT acc[NP];
@@ -395,7 +395,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
   }
 
   const OT rho_c = valid ? at(w, c) : OT(0.0);
-  const double rhob_c = valid ? at(bg + (long long)bRHOB() * N, c) : 1.0;
+  const OT rhob_c = valid ? at(bg + (long long)bRHOB() * N, c) : OT(1.0);
   const AT rt_c = (AT)(rho_c + rhob_c);
 
   // ------------------------------------------------------------ hyperdiffusion + energy terms
@@ -449,7 +449,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
 
   AT ptb = AT(0), bbc[ND];  // background terms, formed in FP64
   if (want_eterm && valid) {
-    double bb2 = 0.0, bbd[ND];
+    OT bb2 = OT(0.0), bbd[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
       bbd[d] = at(bg + (long long)bBB(d) * N, c);

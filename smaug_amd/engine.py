@@ -70,7 +70,7 @@ def load() -> ctypes.CDLL:
     L.smaug_tape_doubles.argtypes = [vp]
     L.smaug_step_taped.argtypes = [vp, vp]
     L.smaug_adjoint_step.argtypes = [vp, vp, vp, vp, vp]
-    L.smaug_tl_begin.argtypes = [vp, dp, dp]
+    L.smaug_tl_begin.argtypes = [vp, dp, dp, dp]
     L.smaug_tl_step.argtypes = [vp, ctypes.c_int]
     L.smaug_tl_get.argtypes = [vp, dp, dp]
     L.smaug_tl_end.argtypes = [vp]
@@ -213,19 +213,26 @@ class Engine:
              "smaug_adjoint_step")
 
     # ---- tangent-linear run (forward-mode derivative; host arrays)
-    def tl_begin(self, dw0: np.ndarray, damap: Optional[np.ndarray] = None):
+    def tl_begin(self, dw0: np.ndarray, dbg: Optional[np.ndarray] = None, damap: Optional[np.ndarray] = None):
         """Starts a tangent-linear run from the current state in the direction
-        dw0 (np x N evolved fields) and, with a driver map set, damap (one
-        value per (j, k) column)."""
+        dw0 (np x N evolved fields), dbg (nb x N background fields e_b, rho_b,
+        b_b; None = 0) and, with a driver map set, damap (one value per
+        (j, k) column)."""
         a = np.ascontiguousarray(dw0, dtype=np.float64).reshape(-1)
         if a.size != self.npert * self.N:
             raise ValueError("dw0 needs np*N values")
+        b = None
+        if dbg is not None:
+            b = np.ascontiguousarray(dbg, dtype=np.float64).reshape(-1)
+            if b.size != (self.nvar - self.npert) * self.N:
+                raise ValueError("dbg needs nb*N values")
         m = None
         if damap is not None:
             m = np.ascontiguousarray(damap, dtype=np.float64).reshape(-1)
             if m.size != self.N // self.rp.n[0]:
                 raise ValueError("damap needs one value per (j, k) column")
-        _chk(load().smaug_tl_begin(self._h, _dp(a), None if m is None else _dp(m)), "smaug_tl_begin")
+        _chk(load().smaug_tl_begin(self._h, _dp(a), None if b is None else _dp(b), None if m is None else _dp(m)),
+             "smaug_tl_begin")
 
     def tl_step(self, n: int = 1):
         _chk(load().smaug_tl_step(self._h, int(n)), "smaug_tl_step")

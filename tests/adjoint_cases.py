@@ -81,6 +81,22 @@ class ReplayFD:
 
         return self._richardson(f, eps)
 
+    def dir_bg(self, dbg: np.ndarray, eps: float) -> float:
+        """Derivative in a direction of the static background fields (nb x N)."""
+        k = npert(self.rp.ndim)
+
+        def f(e):
+            w = self.w0.copy()
+            w[k:] += e * dbg
+            return self._run(w, self.amap)
+
+        return self._richardson(f, eps)
+
+    def dir_state(self, state_of, eps: float) -> float:
+        """Derivative along a curve of initial states: state_of(e) returns the
+        full state (nvar x N) at parameter e, state_of(0) the base state."""
+        return self._richardson(lambda e: self._run(state_of(e), self.amap), eps)
+
     def dir_amap(self, delta: np.ndarray, eps: float) -> float:
         return self._richardson(lambda e: self._run(self.w0, self.amap + e * delta), eps)
 
@@ -103,11 +119,12 @@ def line_loss_grad_np(rp, w: np.ndarray, rhob: np.ndarray, i_line: int) -> np.nd
     return g.reshape(w.shape)
 
 
-def tl_directional(eng, rp, rhob, delta, damap, nsteps: int, i_line: int):
+def tl_directional(eng, rp, rhob, delta, damap, nsteps: int, i_line: int, dbg=None):
     """(sum_n line_loss(w_n), sum_n d line_loss(w_n)) along the engine's
     tangent-linear run from its current state in the direction (delta,
-    damap)."""
-    eng.tl_begin(delta, damap)
+    damap, dbg); rhob is the background density the loss divides by, drhob
+    its direction (row 1 of dbg)."""
+    eng.tl_begin(delta, dbg=dbg, damap=damap)
     loss = dloss = 0.0
     for _ in range(nsteps):
         eng.tl_step(1)
@@ -117,5 +134,7 @@ def tl_directional(eng, rp, rhob, delta, damap, nsteps: int, i_line: int):
         full[3 + 2 * rp.ndim] = rhob  # line_loss_np reads rho_b from the background rows
         loss += line_loss_np(rp, full, i_line)
         dloss += float((line_loss_grad_np(rp, w, rhob, i_line) * dw).sum())
+        if dbg is not None:  # the loss reads rho_t = rho + rho_b: d/d rho_b as d/d rho
+            dloss += float((line_loss_grad_np(rp, w, rhob, i_line)[0] * dbg[1]).sum())
     eng.tl_end()
     return loss, dloss

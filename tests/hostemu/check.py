@@ -231,6 +231,29 @@ def check_tl(case: str, terms: int):
         _, tl = tl_directional(e, rp, rhob, np.zeros((k, rp.ncells)), da, nsteps, il)
         out["amap"] = [tl, fd.dir_amap(da, 1e-6)]
         out["amap_adjoint"] = [tl, float((ga * da).sum())]
+    # a direction of the static background (every field, ghosts included),
+    # relative to each cell's value (the stratified background spans orders
+    # of magnitude)
+    nb = w.shape[0] - k
+    dbg = rng.standard_normal((nb, rp.ncells)) * np.abs(w[k:])
+    _, tl = tl_directional(e, rp, rhob, np.zeros((k, rp.ncells)), None, nsteps, il, dbg=dbg)
+    out["bg"] = [tl, fd.dir_bg(dbg, 1e-6)]
+    # the background's controls: photospheric temperature, transition height
+    if case in ("s2r", "s2e", "s3r"):
+        from smaug_amd import configs
+
+        out["controls"] = {}
+        for name in configs.STATE_CONTROLS:
+            kw = dict(perturb=20261202, amp=1e-3)
+            c0 = {"t_photo": 6000.0, "z_tr": 2.0e6}[name]
+            t = configs.stratified_tangent(rp, name, **kw)
+            _, tl = tl_directional(e, rp, rhob, t[:k], None, nsteps, il, dbg=t[k:])
+
+            def state_of(eps, name=name, c0=c0, kw=kw):
+                return configs.stratified(rp, **kw, **{name: c0 * (1.0 + eps)})[0]
+
+            # d/d(relative change) = c0 d/d(control)
+            out["controls"][name] = [tl * c0, fd.dir_state(state_of, 1e-6)]
     # the run leaves the engine's own state alone
     out["state_untouched"] = bool(np.array_equal(e.state(), s0))
     e.close()

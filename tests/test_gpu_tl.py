@@ -60,6 +60,13 @@ def tl_check(rp, w, pos, nsteps, i_line, per_field=True, seed=7):
                 assert abs(got) <= 1e-12, (i, got)
             else:
                 assert abs(got - want) <= TOL * abs(want), (i, got, want)
+        # a direction of the background fields, relative to each cell's value
+        dbg = rng.standard_normal(w[k:].shape) * np.abs(w[k:])
+        if np.any(dbg):
+            _, got = tl_directional(e, rp, rhob, np.zeros((k, rp.ncells)), None, nsteps, i_line, dbg=dbg)
+            want = fd.dir_bg(dbg, EPS)
+            print(f"  background direction: tangent-linear {got:.12e} fd {want:.12e}", flush=True)
+            assert abs(got - want) <= TOL * abs(want), (got, want)
         # the run's values against the FP64 forward step of the same engine
         e.tl_begin(np.zeros((k, rp.ncells)))
         e.tl_step(nsteps)
@@ -150,3 +157,103 @@ def test_tl_boundary_types(key, rkon):
     nd, types, ft = CASES[key]
     rp, w, pos = bc_case(nd, rkon, types, ft)
     tl_check(rp, w, pos, nsteps=3 if nd == 2 else 2, i_line=7 if nd == 2 else 6)
+
+
+def test_tl_c4_slice_512x256():
+    """The 2D adjoint configuration's grid, 512 (z) x 256 (x), 5 RK4 steps:
+    one direction over all fields against the replayed differences."""
+    from smaug_amd import configs
+
+    rp = configs.stratified_params(ni=(512, 256), ndim=2, rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    tl_check(rp, w, pos, nsteps=5, i_line=340, per_field=False)
+
+
+def test_tl_c5_grid_192cubed_properties():
+    """192^3 (the 3D adjoint configuration's grid), 2 RK4 steps, where the
+    oracle is too slow for differences: the tangents are linear in the
+    direction (t(d1 + 2 d2) = t(d1) + 2 t(d2) to roundoff), scale exactly
+    (bitwise) with a power of two, vanish for a zero direction, and the
+    values equal the FP64 step's state."""
+    from smaug_amd import configs
+    from smaug_amd.engine import Engine
+
+    rp = configs.stratified_params(ni=(192, 192, 192), rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    k = P.npert(3)
+    inner = interior_mask(rp)
+    rng = np.random.default_rng(11)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1)) + 1e-30
+    d1 = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    d2 = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    e = Engine(rp, w, pos)
+    try:
+        def tangent(d, nsteps=2):
+            e.tl_begin(d)
+            e.tl_step(nsteps)
+            wv, dw = e.tl_get()
+            e.tl_end()
+            return wv, dw
+
+        print("[tl] 192^3 RK4: four tangent-linear runs of 2 steps", flush=True)
+        wv, t1 = tangent(d1)
+        _, t2 = tangent(d2)
+        _, t3 = tangent(d1 + 2.0 * d2)
+        lin = np.sqrt(((t3 - (t1 + 2.0 * t2)) ** 2).sum(axis=1) / (t3 ** 2).sum(axis=1))
+        print("  linearity, rel L2 per variable:", lin, flush=True)
+        assert np.all(lin <= 1e-12), lin
+        _, t4 = tangent(4.0 * d1)
+        assert np.array_equal(t4, 4.0 * t1)
+        _, t0 = tangent(np.zeros_like(d1), nsteps=1)
+        assert not np.any(t0)
+        e.step(2)
+        ref = e.state()
+        full = ref.copy()
+        full[:k] = wv
+        err = compare(rp, full, ref)
+        print("  values vs the FP64 step, rel L2 per variable:", err, flush=True)
+        assert max(err) <= 1e-13, err
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("ni,nsteps,i_line", [((16, 12), 4, 7), ((12, 8, 8), 3, 6), ((512, 256), 5, 340)])
+def test_tl_background_controls(ni, nsteps, i_line):
+    """The derivative of the summed line loss with respect to the
+    background's controls -- the photospheric temperature and the transition
+    height of T(z), which set the hydrostatic density, pressure and energy
+    and, through them, the seeded perturbation -- from one tangent-linear
+    run each, in the direction of configs.stratified_tangent, against the
+    oracle rerun from the state rebuilt with the control moved by +-1e-6 and
+    +-2e-6 relative (coefficients replayed).  (512, 256) is the 2D adjoint
+    configuration's grid."""
+    from smaug_amd import configs
+    from smaug_amd.engine import Engine
+
+    nd = len(ni)
+    if ni == (512, 256):
+        rp = configs.stratified_params(ni=ni, ndim=2, rkon=1)
+        kw = dict(perturb=20261202)
+    else:
+        rp, _, _ = strat_case(ni=ni, ndim=nd, rkon=1)
+        kw = dict(perturb=20261202, amp=1e-3)
+    w, pos = configs.stratified(rp, **kw)
+    k = P.npert(nd)
+    print(f"[tl] background controls {nd}D {rp.n} steps={nsteps}", flush=True)
+    e = Engine(rp, w, pos)
+    fd = ReplayFD(oracle, rp, w, pos, None, nsteps, i_line)
+    try:
+        for name, c0 in (("t_photo", 6000.0), ("z_tr", 2.0e6)):
+            t = configs.stratified_tangent(rp, name, **kw)
+            _, got = tl_directional(e, rp, w[3 + 2 * nd], t[:k], None, nsteps, i_line, dbg=t[k:])
+            got *= c0  # per relative change of the control
+
+            def state_of(eps, name=name, c0=c0):
+                return configs.stratified(rp, **kw, **{name: c0 * (1.0 + eps)})[0]
+
+            want = fd.dir_state(state_of, EPS)
+            print(f"  {name}: tangent-linear {got:.12e} fd {want:.12e}", flush=True)
+            assert abs(got - want) <= TOL * abs(want), (name, got, want)
+    finally:
+        fd.close()
+        e.close()

@@ -104,8 +104,11 @@ def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
     """The tangent-linear run (smaug_tl_*): its values reproduce the oracle's
     loss, its directional derivatives of the line loss match the replayed
     differences of the oracle (1e-6) and equal the adjoint's dot products in
-    the same directions (1e-10), per evolved field and for the driver map; the
-    engine's own state is left as it was."""
+    the same directions (1e-10), per evolved field and for the driver map; in
+    a direction of the background fields and along the background's controls
+    (photospheric temperature, transition height: the builder's tangent
+    against the oracle rerun on the rebuilt state) it matches the
+    differences too; the engine's own state is left as it was."""
     r = run(hostlib, "tl", case)
     assert abs(r["loss"][0] - r["loss"][1]) <= 1e-12 * abs(r["loss"][1]), r["loss"]
     for f, pair in enumerate(r["fields"]):
@@ -115,6 +118,12 @@ def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
     if "amap" in r:
         assert close(r["amap"]), r["amap"]
         assert transposed(r["amap_adjoint"]), r["amap_adjoint"]
+    if r["bg"][1] == 0.0:  # no background (Orszag-Tang)
+        assert r["bg"][0] == 0.0, r["bg"]
+    else:
+        assert close(r["bg"]), r["bg"]
+    for name, pair in r.get("controls", {}).items():  # photospheric T, transition height
+        assert close(pair), (name, pair)
     assert r["state_untouched"]
 
 
