@@ -23,6 +23,7 @@ from tests.adjoint_cases import ReplayFD, interior_mask, tl_directional
 from tests.cases import compare, ot_case, strat_case
 
 oracle = pytest.importorskip("oracle.pyoracle")
+torch = pytest.importorskip("torch")  # before the engine library loads: one HIP runtime
 
 pytestmark = pytest.mark.gpu
 
@@ -254,6 +255,49 @@ def test_tl_background_controls(ni, nsteps, i_line):
             want = fd.dir_state(state_of, EPS)
             print(f"  {name}: tangent-linear {got:.12e} fd {want:.12e}", flush=True)
             assert abs(got - want) <= TOL * abs(want), (name, got, want)
+    finally:
+        fd.close()
+        e.close()
+
+
+def test_tl_torch_forward_mode():
+    """smaug_amd.tangent.state_loss under torch's forward-mode AD (dual
+    tensors on the GPU, the loss in torch): the value equals the oracle's
+    loss and the tangent its replayed differences, in a direction of the
+    initial state and of the background together, and of the state alone."""
+    import torch.autograd.forward_ad as fwAD
+
+    from smaug_amd.engine import Engine
+    from smaug_amd.tangent import line_vz2_loss_bg, state_loss
+
+    rp, w, pos = strat_tl_case((16, 12), 2, 1)
+    k, il, nsteps = P.npert(2), 7, 4
+    rng = np.random.default_rng(5)
+    inner = interior_mask(rp)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1))
+    d = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    dbg = rng.standard_normal(w[k:].shape) * np.abs(w[k:])
+    dev = torch.device("cuda:0")
+    w0 = torch.tensor(w[:k].ravel(), device=dev)
+    bg = torch.tensor(w[k:].ravel(), device=dev)
+    e = Engine(rp, w, pos)
+    e.step(3)  # state_loss starts from w0 whatever the engine holds
+    fd = ReplayFD(oracle, rp, w, pos, None, nsteps, il)
+    try:
+        fn = line_vz2_loss_bg(e, il)
+        with fwAD.dual_level():
+            a = fwAD.make_dual(w0, torch.tensor(d.ravel(), device=dev))
+            b = fwAD.make_dual(bg, torch.tensor(dbg.ravel(), device=dev))
+            val, tan = fwAD.unpack_dual(state_loss(e, a, b, nsteps, fn, clock0=(rp.it0, rp.qt0)))
+        want = fd.dir_w0(d, EPS) + fd.dir_bg(dbg, EPS)
+        print(f"  state + background: forward mode {float(tan):.12e} fd {want:.12e}", flush=True)
+        assert abs(float(val) - fd.base) <= 1e-12 * abs(fd.base), (float(val), fd.base)
+        assert abs(float(tan) - want) <= TOL * abs(want), (float(tan), want)
+        with fwAD.dual_level():
+            a = fwAD.make_dual(w0, torch.tensor(d.ravel(), device=dev))
+            tan = fwAD.unpack_dual(state_loss(e, a, bg, nsteps, fn, clock0=(rp.it0, rp.qt0))).tangent
+        want = fd.dir_w0(d, EPS)
+        assert abs(float(tan) - want) <= TOL * abs(want), (float(tan), want)
     finally:
         fd.close()
         e.close()
