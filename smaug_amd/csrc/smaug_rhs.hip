@@ -365,8 +365,10 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
           for (int f = 0; f < NP; f++) F[f] = AT(0);
         }
         const int l0 = (lc[0] + ex[0]) + rx * ((lc[1] + ex[1]) + ry * (lc[2] + ex[2]));
+        // the flux of b_d along d is zero: its slot is neither written nor read
 #pragma unroll
-        for (int f = 0; f < NP; f++) FB[f * L::FN + l0] = F[f];
+        for (int f = 0; f < NP; f++)
+          if (f != fB(ND, d)) FB[f * L::FN + l0] = F[f];
       }
       if (h1) {
         AT F[NP];
@@ -377,7 +379,8 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
           for (int f = 0; f < NP; f++) F[f] = AT(0);
         }
 #pragma unroll
-        for (int f = 0; f < NP; f++) FB[f * L::FN + l1] = F[f];
+        for (int f = 0; f < NP; f++)
+          if (f != fB(ND, d)) FB[f * L::FN + l1] = F[f];
       }
       __syncthreads();
       if (valid && !z4e<EDGE>(G, zFLUX(ND), d, x[d])) {
@@ -386,6 +389,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
         const AT inv = rdx[d] * AT(1.0 / 12.0);
 #pragma unroll
         for (int f = 0; f < NP; f++) {
+          if (f == fB(ND, d)) continue;
           const AT* Fp = FB + f * L::FN + li;
           acc[f] -= ((8 * Fp[s] - 8 * Fp[-s]) + (Fp[-2 * s] - Fp[2 * s])) * inv;
         }
@@ -858,8 +862,9 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
 // FP64: three, the number its 52 KB of LDS per workgroup allows (146
 // VGPRs); stating it lets the scheduler spend registers up to that bound
 // (measured 0.8 % faster than a bound of one workgroup).  Dual (the
-// tangent-linear step, 105 KB of LDS, one workgroup per CU): at most 256
-// registers, the excess spilled to scratch.
+// tangent-linear step): its 105 KB of LDS in 3D leave one workgroup per CU
+// whatever the bound, so the compiler may use the accumulation registers
+// too (256 + 14 at gfx950, no scratch).
 template <int ND, typename SH>
 __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : (sizeof(SH) == 8 ? 3 : 2)) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
