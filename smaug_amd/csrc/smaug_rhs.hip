@@ -864,7 +864,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
 // (measured 0.8 % faster than a bound of one workgroup).  Dual (the
 // tangent-linear step): its 105 KB of LDS in 3D leave one workgroup per CU
 // whatever the bound, so the compiler may use the accumulation registers
-// too (256 + 14 at gfx950, no scratch).
+// too (256 + 10 at gfx950, no scratch).
 template <int ND, typename SH>
 __global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : (sizeof(SH) == 8 ? 3 : 2)) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
