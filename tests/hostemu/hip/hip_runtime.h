@@ -148,4 +148,6 @@ inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 // the hardware reciprocal: exact on the host (the refinement steps that
 // follow it in the kernels then change nothing)
 #define __builtin_amdgcn_rcp(x) (1.0 / (x))
+// wave priority: scheduling only, nothing to emulate
+#define __builtin_amdgcn_s_setprio(p) ((void)0)
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
