@@ -247,6 +247,10 @@ int smaug_load_pert(smaug_t* s, const double* src_device);
 /* ---- adjoint (reverse-mode derivative) of the step ------------------------
  * No counterpart in the reference: the tangent/adjoint step entry points of
  * the differentiable binding.  All buffers are device pointers.
+ * smaug_set_driver_map with a map, smaug_step_taped and smaug_adjoint_step
+ * return SMAUG_E_UNSUPPORTED unless the environment sets SMAUG_ADJOINT=1:
+ * their first MI355X run ended in a GPU fault whose cause is not identified
+ * (they are checked in a host build of the same sources instead).
  *   smaug_tape_doubles  -- size of one step's tape: the stage inputs of the
  *                          step (4*np*N doubles for rkon=1, np*N for rkon=0)
  *                          and 8 doubles of metadata (dt, qt, driver factor).
