@@ -261,6 +261,46 @@ def check_tl(case: str, terms: int):
     return out
 
 
+def check_tl_refusals():
+    """The tangent-linear entry points refuse what they do not cover and run
+    FP64 from a mixed-precision engine."""
+    from smaug_amd.engine import SmaugError
+
+    out = {}
+
+    def refused(name, fn):
+        try:
+            fn()
+            out[name] = "accepted"
+        except SmaugError as err:
+            out[name] = str(err)
+
+    rp, w, pos, _, _, _ = make_case("s2d")  # divB source terms on
+    e = Engine(rp, w, pos)
+    k = P.npert(rp.ndim)
+    refused("divb", lambda: e.tl_begin(np.zeros((k, rp.ncells))))
+    e.close()
+    rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
+    e = Engine(rp, w, pos)
+    refused("step_before_begin", lambda: e.tl_step(1))
+    refused("map_direction_without_map",
+            lambda: e.tl_begin(np.zeros((k, rp.ncells)), damap=np.ones(rp.ncells // rp.n[0])))
+    e.tl_begin(np.zeros((k, rp.ncells)))
+    e.tl_step(2)
+    w64, _ = e.tl_get()
+    e.tl_end()
+    e.close()
+    rp.precision = P.PREC_MIXED
+    e = Engine(rp, w, pos)
+    e.tl_begin(np.zeros((k, rp.ncells)))
+    e.tl_step(2)
+    wm, _ = e.tl_get()
+    e.tl_end()
+    e.close()
+    out["mixed_engine_runs_fp64"] = bool(np.array_equal(w64, wm))
+    return out
+
+
 def check_autograd(rkon: int):
     """smaug_amd.adjoint.driver_loss on CPU tensors (the host build's device
     memory is host memory); torch's stream calls are replaced by no-ops."""
@@ -431,6 +471,8 @@ def main(argv):
         r = check_mixed(argv[1])
     elif what == "adj":
         r = check_adj(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
+    elif what == "tl_refusals":
+        r = check_tl_refusals()
     elif what == "tl":
         r = check_tl(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
     else:
