@@ -149,17 +149,21 @@ def main(argv=None):
     ms_prim, ms_rhs, ms_evstep = eng.time_kernels(args.kernel_steps)
     # tangent-linear step (forward-mode derivative, dual-number stage kernel)
     # from the same state: one untimed step, then timed steps
+    def time_tl(e, label):
+        e.tl_begin(np.zeros((2 + 2 * rp.ndim, rp.ncells)))
+        e.tl_step(1)
+        e.sync()
+        t0 = time.perf_counter()
+        e.tl_step(args.tl_steps)
+        e.sync()
+        ms = (time.perf_counter() - t0) / args.tl_steps * 1e3
+        e.tl_end()
+        print(f"[bench] {label}: timed {args.tl_steps} steps, {ms:.3f} ms/step", file=sys.stderr, flush=True)
+        return ms
+
     tl = None
     if args.tl_steps > 0:
-        eng.tl_begin(np.zeros((2 + 2 * rp.ndim, rp.ncells)))
-        eng.tl_step(1)
-        eng.sync()
-        t0 = time.perf_counter()
-        eng.tl_step(args.tl_steps)
-        eng.sync()
-        ms_tl = (time.perf_counter() - t0) / args.tl_steps * 1e3
-        eng.tl_end()
-        print(f"[bench] tangent-linear: timed {args.tl_steps} steps, {ms_tl:.3f} ms/step", file=sys.stderr, flush=True)
+        ms_tl = time_tl(eng, "tangent-linear")
         tl = {"ms_per_step": ms_tl, "over_forward": ms_tl / ms_step, "steps": args.tl_steps,
               "note": "FP64 tangent-linear step (values and tangents, frozen nu, wave-speed maxima and dt)"}
     eng.close()
@@ -173,6 +177,10 @@ def main(argv=None):
     wallm = timed(engm, "mixed")
     statem = engm.state()
     msm_prim, msm_rhs, _ = engm.time_kernels(args.kernel_steps)
+    if tl is not None:  # the mixed-precision tangent-linear step (FP32 pairs in the stencils)
+        ms_tlm = time_tl(engm, "tangent-linear mixed")
+        tl["mixed_ms_per_step"] = ms_tlm
+        tl["mixed_speedup_vs_fp64"] = tl["ms_per_step"] / ms_tlm
     engm.close()
     k = 2 + 2 * rp.ndim
 

@@ -279,8 +279,10 @@ int smaug_adjoint_step(smaug_t* s, const double* tape_device, const double* wnex
  * current state, the tangents its derivative, with the hyperdiffusion
  * coefficients, the wave-speed maxima and dt held at their values along that
  * trajectory.  The run has its own buffers and clock: the context's state and
- * clock do not change.  Host arrays.  The run is FP64 whatever the
- * context's precision (SMAUG_PREC_MIXED included).  divbon = 1 is rejected.
+ * clock do not change.  Host arrays.  With SMAUG_PREC_MIXED the stencils
+ * run on FP32 pairs (the value and tangent shadows, as the mixed step's FP32
+ * shadows) over FP64 states, else everything is FP64.  divbon = 1 is
+ * rejected.
  *   smaug_tl_begin -- starts a run from the current state in the direction
  *                     dw0 (np*N doubles, field-major; its ghost cells are
  *                     re-filled with the boundary maps), dbg (the static

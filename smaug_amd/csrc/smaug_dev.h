@@ -313,7 +313,9 @@ void launch_rhs(const Geo& G, hipStream_t st, const RhsArgs& A);
 // (value, tangent) arrays, with nu and the wave-speed maxima frozen
 struct Dual;
 void launch_rhs_tl(const Geo& G, hipStream_t st, const RhsArgs& A);
-void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const Dual* bg, Dual* prim,
+// (G.mixed: prim and shadow hold FP32 duals, the shadow of w for the
+// stencils; else prim holds Dual and shadow is unused)
+void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const Dual* bg, void* prim, void* shadow,
                     unsigned long long* cmax, int* flag);
 // d[i] = (v[i], t[i]) (t null: tangent 0), and back (either output may be null)
 void launch_dual_pack(hipStream_t st, long long n, const double* v, const double* t, Dual* d);

@@ -100,7 +100,8 @@ struct smaug_ctx {
   // tangent-linear run (smaug_tl_*): Dual stage buffers, derived fields
   // and driver map, its own maxima slots, (dt, t), clock and host staging
   Dual* tlU[4] = {nullptr, nullptr, nullptr, nullptr};
-  Dual* tlprim = nullptr;
+  void* tlprim = nullptr;     // Dual, or DualF in a mixed-precision context
+  void* tlshadow = nullptr;   // mixed precision: FP32 dual shadow of the stage input
   Dual* tlbg = nullptr;  // background with its tangent
   Dual* tlamap = nullptr;
   unsigned long long* tlcmax = nullptr;  // 2 slots of 4
@@ -117,12 +118,14 @@ static void free_tl(smaug_t* s) {
     u = nullptr;
   }
   hipFree(s->tlprim);
+  hipFree(s->tlshadow);
   hipFree(s->tlbg);
   hipFree(s->tlamap);
   hipFree(s->tlcmax);
   hipFree(s->tldts);
   hipFree(s->tltmp);
-  s->tlprim = s->tlbg = s->tlamap = nullptr;
+  s->tlprim = s->tlshadow = nullptr;
+  s->tlbg = s->tlamap = nullptr;
   s->tlcmax = nullptr;
   s->tldts = s->tltmp = nullptr;
   s->tl = false;
@@ -922,7 +925,9 @@ extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* dbg, 
   const int nbuf = s->p.rkon == 1 ? 4 : 2;
   bool ok = true;
   for (int b = 0; b < nbuf; b++) ok = ok && hipMalloc(&s->tlU[b], sizeof(Dual) * (size_t)F) == hipSuccess;
-  ok = ok && hipMalloc(&s->tlprim, sizeof(Dual) * (size_t)(s->npr * s->N)) == hipSuccess;
+  const size_t pt = s->G.mixed ? sizeof(DualF) : sizeof(Dual);
+  ok = ok && hipMalloc(&s->tlprim, pt * (size_t)(s->npr * s->N)) == hipSuccess;
+  if (s->G.mixed) ok = ok && hipMalloc(&s->tlshadow, sizeof(DualF) * (size_t)F) == hipSuccess;
   ok = ok && hipMalloc(&s->tlbg, sizeof(Dual) * (size_t)(s->nb * s->N)) == hipSuccess;
   if (s->amap) ok = ok && hipMalloc(&s->tlamap, sizeof(Dual) * (size_t)ncol) == hipSuccess;
   ok = ok && hipMalloc(&s->tlcmax, sizeof(unsigned long long) * 8) == hipSuccess;
@@ -969,12 +974,12 @@ static void tl_stage(smaug_t* s, const Geo& Gt, Dual* win, const Dual* base, Dua
                      bool first, const Dual* r1, const Dual* r2, const Dual* r3) {
   hipStream_t st = s->stream;
   const int slot = s->tlslot;
-  launch_prim_tl(Gt, st, win, s->tlbg, s->tlprim, s->tlcmax + 4 * slot, s->flag);
+  launch_prim_tl(Gt, st, win, s->tlbg, s->tlprim, s->tlshadow, s->tlcmax + 4 * slot, s->flag);
   if (first)
     launch_dt(st, s->tlcmax + 4 * slot, s->nd, s->p.dx, s->p.courant, s->p.dt, s->p.moddton, s->tldts);
   RhsArgs A;
   A.win = (const double*)win;
-  A.wsh = win;
+  A.wsh = Gt.mixed ? (const void*)s->tlshadow : (const void*)win;
   A.bg = (const double*)s->tlbg;
   A.prim = s->tlprim;
   A.delx = s->delx;
@@ -1001,8 +1006,7 @@ static void tl_stage(smaug_t* s, const Geo& Gt, Dual* win, const Dual* base, Dua
 extern "C" int smaug_tl_step(smaug_t* s, int nsteps) {
   if (!s || nsteps < 0) return fail(SMAUG_E_ARG, "bad arguments to smaug_tl_step");
   if (!s->tl) return fail(SMAUG_E_ARG, "no tangent-linear run (smaug_tl_begin)");
-  Geo Gt = s->G;
-  Gt.mixed = 0;
+  const Geo& Gt = s->G;  // mixed: FP32 dual stencils over FP64 dual states
   for (int n = 0; n < nsteps; n++) {
     s->tlqt = s->tlqt + s->p.dt;  // as the forward step (quirk Q5)
     const double tdep = driver_tdep_at(s, s->tlit, s->tlqt);

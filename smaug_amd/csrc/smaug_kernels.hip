@@ -136,10 +136,12 @@ __global__ __launch_bounds__(256) void k_prim(Geo G, const double* __restrict__ 
 // tangent of every evolved and background field); v and q_e are formed in Dual arithmetic
 // (the same expressions as k_prim), the wave speeds and their maxima from
 // the values alone (frozen, as the hyperdiffusion coefficients they scale)
-template <int ND>
+// PT = DualF (the mixed-precision run): prim rounded to FP32 duals, and the
+// FP32 dual shadow of the state for the stencils
+template <int ND, typename PT>
 __global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__ w, const Dual* __restrict__ bg,
-                                                 Dual* __restrict__ prim, unsigned long long* __restrict__ cmax_bits,
-                                                 int* __restrict__ flag) {
+                                                 PT* __restrict__ prim, PT* __restrict__ shadow,
+                                                 unsigned long long* __restrict__ cmax_bits, int* __restrict__ flag) {
   const long long N = G.N;
   double mx[3] = {0.0, 0.0, 0.0};
   int bad = 0;
@@ -162,8 +164,17 @@ __global__ __launch_bounds__(256) void k_prim_tl(Geo G, const Dual* __restrict__
       b2 = b2 + b[d] * b[d];
     }
 #pragma unroll
-    for (int d = 0; d < ND; d++) prim[(long long)pV(d) * N + c] = m[d] / rt;
-    prim[(long long)pQE(ND) * N + c] = e - 0.5 * (b2 + mm / rt);
+    for (int d = 0; d < ND; d++) prim[(long long)pV(d) * N + c] = (PT)(m[d] / rt);
+    prim[(long long)pQE(ND) * N + c] = (PT)(e - 0.5 * (b2 + mm / rt));
+    if (shadow) {
+      shadow[c] = (PT)rho;
+      shadow[(long long)fE(ND) * N + c] = (PT)e;
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        shadow[(long long)fMOM(d) * N + c] = (PT)m[d];
+        shadow[(long long)fB(ND, d) * N + c] = (PT)b[d];
+      }
+    }
     // wave speeds of the values (k_prim's expressions)
     const double gm = G.gamma, rtv = rt.v;
     double mmv = m[0].v * m[0].v, b2v = b[0].v * b[0].v, bb2 = bb[0].v * bb[0].v, bbb = bb[0].v * b[0].v;
@@ -318,14 +329,22 @@ void launch_prim(const Geo& G, hipStream_t st, const double* w, const double* bg
   }
 }
 
-void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const Dual* bg, Dual* prim,
+void launch_prim_tl(const Geo& G, hipStream_t st, const Dual* w, const Dual* bg, void* prim, void* shadow,
                     unsigned long long* cmax, int* flag) {
   long long nb = (G.N + 255) / 256;
   if (nb > 2048) nb = 2048;
-  if (G.nd == 3)
-    hipLaunchKernelGGL(k_prim_tl<3>, dim3((unsigned)nb), dim3(256), 0, st, G, w, bg, prim, cmax, flag);
-  else
-    hipLaunchKernelGGL(k_prim_tl<2>, dim3((unsigned)nb), dim3(256), 0, st, G, w, bg, prim, cmax, flag);
+  const dim3 g((unsigned)nb), b(256);
+  if (G.mixed) {
+    if (G.nd == 3)
+      hipLaunchKernelGGL((k_prim_tl<3, DualF>), g, b, 0, st, G, w, bg, (DualF*)prim, (DualF*)shadow, cmax, flag);
+    else
+      hipLaunchKernelGGL((k_prim_tl<2, DualF>), g, b, 0, st, G, w, bg, (DualF*)prim, (DualF*)shadow, cmax, flag);
+  } else {
+    if (G.nd == 3)
+      hipLaunchKernelGGL((k_prim_tl<3, Dual>), g, b, 0, st, G, w, bg, (Dual*)prim, (Dual*)nullptr, cmax, flag);
+    else
+      hipLaunchKernelGGL((k_prim_tl<2, Dual>), g, b, 0, st, G, w, bg, (Dual*)prim, (Dual*)nullptr, cmax, flag);
+  }
 }
 
 void launch_dual_pack(hipStream_t st, long long n, const double* v, const double* t, Dual* d) {

@@ -92,8 +92,38 @@ __device__ inline Dual rcp_pos(Dual x) {
   return Dual(r, -x.t * r * r);
 }
 
+// FP32 dual number: the stencils of the mixed-precision tangent-linear step
+// (value and tangent rounded to FP32, as the FP32 shadows of the mixed
+// forward step), over an FP64 Dual own-cell state
+struct DualF {
+  float v, t;
+  DualF() = default;
+  __host__ __device__ constexpr DualF(float a) : v(a), t(0.0f) {}
+  __host__ __device__ constexpr DualF(float a, float b) : v(a), t(b) {}
+  __host__ __device__ explicit DualF(double a) : v((float)a), t(0.0f) {}
+  __host__ __device__ explicit DualF(int a) : v((float)a), t(0.0f) {}
+  __host__ __device__ explicit DualF(const Dual& d) : v((float)d.v), t((float)d.t) {}
+  __host__ __device__ explicit operator Dual() const { return Dual(v, t); }
+};
+__host__ __device__ inline DualF operator+(DualF a, DualF b) { return DualF(a.v + b.v, a.t + b.t); }
+__host__ __device__ inline DualF operator-(DualF a, DualF b) { return DualF(a.v - b.v, a.t - b.t); }
+__host__ __device__ inline DualF operator-(DualF a) { return DualF(-a.v, -a.t); }
+__host__ __device__ inline DualF operator*(DualF a, DualF b) { return DualF(a.v * b.v, a.t * b.v + a.v * b.t); }
+__host__ __device__ inline DualF operator/(DualF a, DualF b) {
+  const float q = a.v / b.v;
+  return DualF(q, (a.t - q * b.t) / b.v);
+}
+__host__ __device__ inline DualF operator*(DualF a, float b) { return DualF(a.v * b, a.t * b); }
+__host__ __device__ inline DualF operator*(float a, DualF b) { return DualF(a * b.v, a * b.t); }
+__host__ __device__ inline DualF& operator+=(DualF& a, DualF b) { return a = a + b; }
+__host__ __device__ inline DualF& operator-=(DualF& a, DualF b) { return a = a - b; }
+__device__ inline DualF rcp_pos(DualF x) {
+  const float r = rcp_pos(x.v);
+  return DualF(r, -x.t * r * r);
+}
+
 // own-cell type of the stage kernel for a stencil type: FP64 for the FP64
-// and mixed-precision stencils, Dual for the tangent-linear step
+// and mixed-precision stencils, Dual for the tangent-linear steps
 template <typename SH>
 struct OwnT {
   using T = double;
@@ -102,10 +132,15 @@ template <>
 struct OwnT<Dual> {
   using T = Dual;
 };
+template <>
+struct OwnT<DualF> {
+  using T = Dual;
+};
 // a staged value in the precision sums with the background take
 __device__ inline double widen(float x) { return x; }
 __device__ inline double widen(double x) { return x; }
 __device__ inline Dual widen(Dual x) { return x; }
+__device__ inline Dual widen(DualF x) { return Dual(x.v, x.t); }
 
 template <int ND>
 __device__ inline bool in_grid(const Geo& G, const int* x) {
@@ -218,6 +253,17 @@ __device__ inline void nu_win<Dual, Dual>(const Dual* __restrict__ qb, int ci, i
   nu7<double>(Q, dxk.v, r, l);
   nur = Dual(r);
   nul = Dual(l);
+}
+template <>
+__device__ inline void nu_win<DualF, DualF>(const DualF* __restrict__ qb, int ci, int s, DualF dxk, DualF& nur,
+                                            DualF& nul) {
+  float Q[7];
+#pragma unroll
+  for (int o = 0; o < 7; o++) Q[o] = qb[ci + (o - 3) * s].v;
+  float r, l;
+  nu7<float>(Q, dxk.v, r, l);
+  nur = DualF(r);
+  nul = DualF(l);
 }
 
 // fluxes of every evolved field along d at cell cc (src/centdiff1.cu:51-256,

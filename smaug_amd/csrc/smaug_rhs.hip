@@ -24,6 +24,8 @@
 // RK combination are FP64 in both.
 #include <math.h>
 
+#include <type_traits>
+
 #include "../../include/smaug_hip.h"
 #include "smaug_dev.h"
 #include "smaug_ops.h"
@@ -871,9 +873,10 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
 // (measured 0.8 % faster than a bound of one workgroup).  Dual (the
 // tangent-linear step): its 105 KB of LDS in 3D leave one workgroup per CU
 // whatever the bound, so the compiler may use the accumulation registers
-// too (256 + 10 at gfx950, no scratch).
+// too (256 + 10 at gfx950, no scratch); DualF (its mixed-precision form, FP32
+// pairs in 52 KB of LDS): two wavefronts per SIMD.
 template <int ND, typename SH>
-__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : (sizeof(SH) == 8 ? 3 : 2)) void k_rhs_tiled(Geo G, RhsArgs A) {
+__global__ __launch_bounds__(256, sizeof(SH) == 4 ? 5 : (std::is_same<SH, double>::value ? 3 : 2)) void k_rhs_tiled(Geo G, RhsArgs A) {
   using L = Lay<ND>;
   constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
   __shared__ SH sm[L::SMEM];
@@ -924,13 +927,20 @@ static void launch_tiled_t(const Geo& G, hipStream_t st, const RhsArgs& A) {
   hipLaunchKernelGGL((k_rhs_tiled<ND, SH>), dim3(nb), dim3(256), 0, st, G, A);
 }
 
-// the tangent-linear stage: every state array of A (win, wsh, prim, base,
-// r1..r3, wout and the driver map) holds Dual values
+// the tangent-linear stage: every state array of A (win, base, r1..r3,
+// wout, bg and the driver map) holds Dual values, and wsh and prim hold Dual
+// values too, or FP32 DualF values when G.mixed
 void launch_rhs_tl(const Geo& G, hipStream_t st, const RhsArgs& A) {
-  if (G.nd == 3)
+  if (G.mixed) {  // FP32 dual stencils (wsh, prim), FP64 dual own cells
+    if (G.nd == 3)
+      launch_tiled_t<3, DualF>(G, st, A);
+    else
+      launch_tiled_t<2, DualF>(G, st, A);
+  } else if (G.nd == 3) {
     launch_tiled_t<3, Dual>(G, st, A);
-  else
+  } else {
     launch_tiled_t<2, Dual>(G, st, A);
+  }
 }
 
 void launch_rhs_tiled(const Geo& G, hipStream_t st, const RhsArgs& A) {

@@ -333,3 +333,73 @@ def test_tl_slice_horizon_controls(control):
     finally:
         fd.close()
         e.close()
+
+
+def _tl_runs(rp, w, pos, d, nsteps, i_line):
+    """(loss, derivative, final state, final tangent) of the FP64 and the
+    mixed-precision tangent-linear runs in direction d."""
+    from smaug_amd.engine import Engine
+
+    out = []
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        rp.precision = prec
+        e = Engine(rp, w, pos)
+        try:
+            loss, dl = tl_directional(e, rp, w[3 + 2 * rp.ndim], d, None, nsteps, i_line)
+            e.tl_begin(d)
+            e.tl_step(nsteps)
+            wv, dw = e.tl_get()
+            e.tl_end()
+        finally:
+            e.close()
+        out.append((loss, dl, wv, dw))
+    rp.precision = P.PREC_FP64
+    return out
+
+
+def _rel(a, b):
+    return np.sqrt(((a - b) ** 2).sum(axis=1) / (b ** 2).sum(axis=1))
+
+
+@pytest.mark.parametrize("ni,i_line", [((16, 12), 7), ((12, 8, 8), 6), ((30, 17, 37), 9)])
+def test_tl_mixed_precision_against_fp64(ni, i_line):
+    """The mixed-precision tangent-linear run (FP32 value and tangent pairs in
+    the stencils, FP64 dual states) against the FP64 one over 10 RK4 steps:
+    states and tangents per variable and the line loss's derivative within
+    1e-5 relative, and not identical (the FP32 pairs are in use)."""
+    rp, w, pos = strat_tl_case(ni, len(ni), 1)
+    k = P.npert(rp.ndim)
+    inner = interior_mask(rp)
+    rng = np.random.default_rng(13)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1))
+    d = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    (l64, d64, w64, t64), (lm, dm, wm, tm) = _tl_runs(rp, w, pos, d, 10, i_line)
+    ev, et = _rel(wm, w64), _rel(tm, t64)
+    print(f"[tl mixed] {ni}: states {ev.max():.2e}, tangents {et.max():.2e}, derivative {abs(dm - d64) / abs(d64):.2e}",
+          flush=True)
+    assert ev.max() <= 1e-5 and et.max() <= 1e-5, (ev, et)
+    assert abs(dm - d64) <= 1e-5 * abs(d64), (dm, d64)
+    assert et.max() > 0
+
+
+def test_tl_mixed_c5_grid_192cubed():
+    """The 3D adjoint configuration (192^3, FP32 shadows over an FP64 state):
+    the derivative of the summed v_z^2 on the plane z ~ 4 Mm over 5 RK4 steps
+    in a random direction of the initial state, from the mixed-precision
+    tangent-linear run, against the FP64 tangent-linear reference (1e-5)."""
+    from smaug_amd import configs
+
+    rp = configs.stratified_params(ni=(192, 192, 192), rkon=1)
+    w, pos = configs.stratified(rp, perturb=20261202)
+    k = P.npert(3)
+    inner = interior_mask(rp)
+    rng = np.random.default_rng(17)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1))
+    d = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    i_line = 2 + int(round((4.0e6 - 0.1e6) / (5.9e6 / 192)))
+    (l64, d64, _, t64), (lm, dm, _, tm) = _tl_runs(rp, w, pos, d, 5, i_line)
+    et = _rel(tm, t64)
+    print(f"[tl mixed] 192^3: loss {abs(lm - l64) / abs(l64):.2e}, derivative {abs(dm - d64) / abs(d64):.2e}, "
+          f"tangents {et.max():.2e}", flush=True)
+    assert abs(dm - d64) <= 1e-5 * abs(d64), (dm, d64)
+    assert et.max() <= 1e-5, et

@@ -129,13 +129,16 @@ def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
 
 def test_tangent_linear_refusals_and_precision(hostlib):
     """The tangent-linear run refuses the divB source terms (-5), a step
-    before its start and a driver-map direction without a map (-1), and from
-    a mixed-precision engine runs FP64 (the same values as from an FP64 one)."""
+    before its start and a driver-map direction without a map (-1); from a
+    mixed-precision engine (FP32 pairs in the stencils) its values and
+    tangents stay within 1e-5 of the FP64 run's after 10 steps, and are not
+    identical to them."""
     r = run(hostlib, "tl_refusals")
     assert "failed (-5)" in r["divb"], r
     assert "failed (-1)" in r["step_before_begin"], r
     assert "failed (-1)" in r["map_direction_without_map"], r
-    assert r["mixed_engine_runs_fp64"], r
+    for key in ("mixed_values_rel_l2", "mixed_tangents_rel_l2"):
+        assert max(r[key]) <= 1e-5 and max(r[key]) > 0, (key, r[key])
 
 
 BC_CASES = ["bc:2d_symm_cont:1", "bc:2d_fixed_asymm:0", "bc:3d_cont_symm_fixed:1", "bc:3d_asymm_fixed_cont:1"]
