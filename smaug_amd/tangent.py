@@ -10,7 +10,8 @@ ghost cells included) and of the static background bg (nb*N values: e_b,
 rho_b, b_b).  loss_fn is a torch function of one state's evolved fields and
 of the background (flat float64 tensors on w0's device).  The
 hyperdiffusion coefficients, the wave-speed maxima and dt are held at their
-values along the trajectory, as in the reverse mode (smaug_amd.adjoint).
+values along the trajectory, as in the reverse mode (smaug_amd.adjoint).  In
+a mixed-precision engine the run's stencils use FP32 value/tangent pairs.
 
 States pass through host memory (smaug_set_state / smaug_get_state and the
 tangent-linear run's host arrays) on the engine's own stream; one jvp costs
@@ -51,7 +52,7 @@ def _host(t: torch.Tensor) -> np.ndarray:
 
 
 def _run(eng: Engine, w0: np.ndarray, bg: np.ndarray, clock0, nsteps: int, loss_fn: LossFn, dev,
-         dw0=None, dbg=None) -> Tuple[float, float]:
+         dw0=None, dbg=None) -> Tuple[torch.Tensor, torch.Tensor]:
     """(L, dL): the forward loop from (w0, bg), and with a direction the
     tangent-linear run alongside it."""
     k = eng.npert
