@@ -282,7 +282,9 @@ def main(argv=None):
             "adjoint_over_forward": None,
             "status": "not measured: the adjoint entry points are gated off on the GPU (SMAUG_ADJOINT) "
                       "until the GPU fault of their first MI355X run is explained; they are checked in "
-                      "the host build (tests/test_hostemu.py)",
+                      "the host build (tests/test_hostemu.py), where their dot products equal the "
+                      "tangent-linear run's to roundoff; forward-mode derivatives run on the GPU "
+                      "(tangent_linear above)",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
