@@ -289,11 +289,17 @@ int smaug_adjoint_step(smaug_t* s, const double* tape_device, const double* wnex
  *                     background fields, nb*N doubles: e_b, rho_b, b_b per
  *                     cell, ghosts included; NULL = 0) and, when a driver
  *                     amplitude map is set, damap (n1*n2 values, NULL = 0).
+ *   smaug_tl_driver_amplitude -- adds to the run's direction the tangent
+ *                     d_amplitude of the driver's amplitude drv[0] (vertical,
+ *                     torsional and spicule drivers; 0 at smaug_tl_begin)
+ *                     for the steps that follow; refused without a driver or
+ *                     with a driver amplitude map (the map is the amplitude).
  *   smaug_tl_step  -- advances the run by nsteps steps.
  *   smaug_tl_get   -- copies the run's state (w, np*N) and its tangent (dw,
  *                     np*N); either may be NULL.
  *   smaug_tl_end   -- frees the run (smaug_destroy does too). */
 int smaug_tl_begin(smaug_t* s, const double* dw0_host, const double* dbg_host, const double* damap_host);
+int smaug_tl_driver_amplitude(smaug_t* s, double d_amplitude);
 int smaug_tl_step(smaug_t* s, int nsteps);
 int smaug_tl_get(smaug_t* s, double* w_host, double* dw_host);
 int smaug_tl_end(smaug_t* s);

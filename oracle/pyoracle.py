@@ -41,6 +41,7 @@ def lib():
         L.orc_probe_grad.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_probe_visc.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         L.orc_set_driver_map.argtypes = [vp, dp]
+        L.orc_set_driver_param.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.orc_restart.argtypes = [vp, dp, ctypes.c_int, ctypes.c_double]
         L.orc_nu_mode.argtypes = [vp, ctypes.c_int]
         L.orc_nu_mode.restype = ctypes.c_long
@@ -133,6 +134,11 @@ class Oracle:
         if a.size != n[1] * (n[2] if self.nd == 3 else 1):
             raise ValueError("driver map needs one value per (j, k) column")
         lib().orc_set_driver_map(self._h, _dp(a))
+
+    def set_driver_param(self, q: int, v: float):
+        """Driver parameter drv[q] for the steps that follow."""
+        if lib().orc_set_driver_param(self._h, int(q), float(v)) != 0:
+            raise ValueError("driver parameter index out of range")
 
     def restart(self, w: np.ndarray, it: int = 0, qt: float = 0.0):
         """Start again from state w (nvar x N) at step it and clock qt."""

@@ -1530,6 +1530,14 @@ int orc_set_driver_map(orc_t* o, const double* amap) {
   return 0;
 }
 
+/* driver parameter drv[q] for the steps that follow (the driver sources
+ * read p.drv at every step); differences in the driver amplitude */
+int orc_set_driver_param(orc_t* o, int q, double v) {
+  if (q < 0 || q >= SMAUG_NDRV) return -1;
+  o->p.drv[q] = v;
+  return 0;
+}
+
 /* nu tape: mode 0 off, 1 record (clears the tape), 2 replay from the start.
  * Returns the number of recorded (nu_r, nu_l) pairs. */
 long orc_nu_mode(orc_t* o, int mode) {

@@ -18,6 +18,7 @@ int orc_probe_grad(orc_t* o, int kind, const double* a, int bcidx, int dir, doub
 int orc_probe_visc(orc_t* o, int field, int dim, double* nur, double* nul, double* cmax);
 int orc_restart(orc_t* o, const double* w, int it, double qt);
 int orc_set_driver_map(orc_t* o, const double* amap);
+int orc_set_driver_param(orc_t* o, int q, double v);
 long orc_nu_mode(orc_t* o, int mode);
 int orc_num_threads(void);
 void orc_destroy(orc_t* o);

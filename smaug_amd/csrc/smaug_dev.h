@@ -236,6 +236,8 @@ struct RhsArgs {
   const double* dts;                    // device dt
   double hfrac;                         // stage step = hfrac*dt
   double tdep;                          // driver time factor sin(2 pi qt/P)
+  double damp;                          // tangent of the driver amplitude drv[0]
+                                        // (tangent-linear run; else 0)
   const double* amap;                   // driver amplitude per (j,k) column, or null
   const double* dshape;                 // vertical driver on a separable grid: its
                                         // factors exp(-dz^2/sz^2) over dim 0, then

@@ -109,6 +109,7 @@ struct smaug_ctx {
   double* tltmp = nullptr;  // 2 np*N doubles
   int tlcur = 0, tlslot = 0, tlit = 0;
   double tlqt = 0;
+  double tldamp = 0;  // tangent of the driver amplitude drv[0]
   bool tl = false;
 };
 
@@ -451,6 +452,7 @@ static RhsArgs rhs_args(smaug_t* s, const double* win, const double* base, doubl
   A.dts = s->dts;
   A.hfrac = hfrac;
   A.tdep = tdep;
+  A.damp = 0.0;
   A.amap = s->amap;
   A.dshape = s->dshape;
   return A;
@@ -966,7 +968,20 @@ extern "C" int smaug_tl_begin(smaug_t* s, const double* dw0, const double* dbg, 
   s->tlslot = 0;
   s->tlit = s->it;
   s->tlqt = s->qt;
+  s->tldamp = 0.0;
   s->tl = true;
+  return SMAUG_OK;
+}
+
+extern "C" int smaug_tl_driver_amplitude(smaug_t* s, double d_amplitude) {
+  if (!s) return fail(SMAUG_E_ARG, "null context");
+  if (!s->tl) return fail(SMAUG_E_ARG, "no tangent-linear run (smaug_tl_begin)");
+  if (s->p.driver == SMAUG_DRIVER_NONE && d_amplitude != 0.0)
+    return fail(SMAUG_E_ARG, "no driver: its amplitude has no direction");
+  if (s->amap && d_amplitude != 0.0)
+    return fail(SMAUG_E_ARG, "with a driver amplitude map the amplitude is the map: give its direction to smaug_tl_begin");
+  if (!isfinite(d_amplitude)) return fail(SMAUG_E_ARG, "the amplitude direction is not finite");
+  s->tldamp = d_amplitude;
   return SMAUG_OK;
 }
 
@@ -997,6 +1012,7 @@ static void tl_stage(smaug_t* s, const Geo& Gt, Dual* win, const Dual* base, Dua
   A.dts = s->tldts;
   A.hfrac = hfrac;
   A.tdep = tdep;
+  A.damp = s->tldamp;
   A.amap = (const double*)s->tlamap;
   A.dshape = s->dshape;
   launch_rhs_tl(Gt, st, A);

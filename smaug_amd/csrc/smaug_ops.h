@@ -136,6 +136,14 @@ template <>
 struct OwnT<DualF> {
   using T = Dual;
 };
+// A value formed from a parameter: with the parameter's tangent t when the
+// own-cell type carries tangents (tangent-linear run), else the value alone.
+template <typename OT>
+__device__ inline OT with_tangent(double v, double t);
+template <>
+__device__ inline double with_tangent<double>(double v, double) { return v; }
+template <>
+__device__ inline Dual with_tangent<Dual>(double v, double t) { return Dual(v, t); }
 // a staged value in the precision sums with the background take
 __device__ inline double widen(float x) { return x; }
 __device__ inline double widen(double x) { return x; }

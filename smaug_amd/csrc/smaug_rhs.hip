@@ -783,7 +783,7 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     // amplitude map: one amplitude per (j,k) column in place of
     // A exp(-dx^2/sxy^2 [- dy^2/sxy^2])
     const OT vz = A.amap ? ((const OT*)A.amap)[x[1] + (ND == 3 ? (long long)G.n[1] * x[2] : 0)] * ez * A.tdep
-                         : OT(dv[0] * exyz * A.tdep);
+                         : with_tangent<OT>(dv[0] * exyz * A.tdep, A.damp * exyz * A.tdep);
     const OT rto = out[fRHO()] + rhob_c;
     out[fMOM(0)] += h * vz * rto;
     out[fE(ND)] += h * (vz * vz) * rto / 2.0;
@@ -798,8 +798,8 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     const double exx = exp(-xp * xp / (dv[6] * dv[6]));
     const double exy = exp(-yp * yp / (dv[7] * dv[7]));
     const double exyz = exx * exy * exz;
-    const double vx = (dv[0] * yp / dv[9]) * exyz * A.tdep;
-    const double vy = -(dv[0] * xp / dv[8]) * exyz * A.tdep;
+    const OT vx = with_tangent<OT>((dv[0] * yp / dv[9]) * exyz * A.tdep, (A.damp * yp / dv[9]) * exyz * A.tdep);
+    const OT vy = with_tangent<OT>(-(dv[0] * xp / dv[8]) * exyz * A.tdep, -(A.damp * xp / dv[8]) * exyz * A.tdep);
     const OT rto = out[fRHO()] + rhob_c;
     out[fMOM(1)] += h * vx * rto;
     out[fMOM(2)] += h * vy * rto;
@@ -812,7 +812,8 @@ __device__ __forceinline__ void rhs_tile(const Geo& G, const RhsArgs& A, SH* __r
     const double xp = G.pxmin[0] + (double)x[0] * G.pdx[0];
     const double yp = G.pxmin[1] + (double)x[1] * G.pdx[1];
     const double r1 = (xp - dv[2]) * (xp - dv[2]), r2 = (yp - dv[3]) * (yp - dv[3]);
-    const double vvz = dv[0] * (exp(-r2 / (dv[6] * dv[6])) * exp(-r1 / (dv[5] * dv[5]))) * A.tdep;
+    const double g = exp(-r2 / (dv[6] * dv[6])) * exp(-r1 / (dv[5] * dv[5]));
+    const OT vvz = with_tangent<OT>(dv[0] * g * A.tdep, A.damp * g * A.tdep);
     out[fMOM(0)] = out[fMOM(0)] + h * vvz * (out[fRHO()] + rhob_c);
   }
   if (A.r1) {

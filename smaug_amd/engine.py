@@ -25,7 +25,7 @@ EXPORTS = [
     "smaug_ini_read", "smaug_ini_write", "smaug_out_write", "smaug_write_out",
     "smaug_set_driver_map", "smaug_set_stream", "smaug_set_clock", "smaug_save_pert", "smaug_load_pert",
     "smaug_tape_doubles", "smaug_step_taped", "smaug_adjoint_step", "smaug_cfl_noalloc",
-    "smaug_tl_begin", "smaug_tl_step", "smaug_tl_get", "smaug_tl_end",
+    "smaug_tl_begin", "smaug_tl_driver_amplitude", "smaug_tl_step", "smaug_tl_get", "smaug_tl_end",
 ]
 
 
@@ -71,6 +71,7 @@ def load() -> ctypes.CDLL:
     L.smaug_step_taped.argtypes = [vp, vp]
     L.smaug_adjoint_step.argtypes = [vp, vp, vp, vp, vp]
     L.smaug_tl_begin.argtypes = [vp, dp, dp, dp]
+    L.smaug_tl_driver_amplitude.argtypes = [vp, ctypes.c_double]
     L.smaug_tl_step.argtypes = [vp, ctypes.c_int]
     L.smaug_tl_get.argtypes = [vp, dp, dp]
     L.smaug_tl_end.argtypes = [vp]
@@ -233,6 +234,10 @@ class Engine:
                 raise ValueError("damap needs one value per (j, k) column")
         _chk(load().smaug_tl_begin(self._h, _dp(a), None if b is None else _dp(b), None if m is None else _dp(m)),
              "smaug_tl_begin")
+
+    def tl_driver_amplitude(self, d: float):
+        """Tangent of the driver amplitude drv[0] in the current run's direction."""
+        _chk(load().smaug_tl_driver_amplitude(self._h, float(d)), "smaug_tl_driver_amplitude")
 
     def tl_step(self, n: int = 1):
         _chk(load().smaug_tl_step(self._h, int(n)), "smaug_tl_step")

@@ -97,6 +97,20 @@ class ReplayFD:
         full state (nvar x N) at parameter e, state_of(0) the base state."""
         return self._richardson(lambda e: self._run(state_of(e), self.amap), eps)
 
+    def dir_amplitude(self, eps: float) -> float:
+        """Derivative per relative change of the driver amplitude drv[0]
+        (A dL/dA), the oracle's parameter moved for the replayed runs."""
+        a0 = float(self.rp.drv[0])
+
+        def f(e):
+            self.o.set_driver_param(0, a0 * (1.0 + e))
+            try:
+                return self._run(self.w0, self.amap)
+            finally:
+                self.o.set_driver_param(0, a0)
+
+        return self._richardson(f, eps)
+
     def dir_amap(self, delta: np.ndarray, eps: float) -> float:
         return self._richardson(lambda e: self._run(self.w0, self.amap + e * delta), eps)
 
@@ -119,12 +133,14 @@ def line_loss_grad_np(rp, w: np.ndarray, rhob: np.ndarray, i_line: int) -> np.nd
     return g.reshape(w.shape)
 
 
-def tl_directional(eng, rp, rhob, delta, damap, nsteps: int, i_line: int, dbg=None):
+def tl_directional(eng, rp, rhob, delta, damap, nsteps: int, i_line: int, dbg=None, damp: float = 0.0):
     """(sum_n line_loss(w_n), sum_n d line_loss(w_n)) along the engine's
     tangent-linear run from its current state in the direction (delta,
-    damap, dbg); rhob is the background density the loss divides by, drhob
-    its direction (row 1 of dbg)."""
+    damap, dbg, damp); rhob is the background density the loss divides by,
+    drhob its direction (row 1 of dbg); damp the driver amplitude's."""
     eng.tl_begin(delta, dbg=dbg, damap=damap)
+    if damp:
+        eng.tl_driver_amplitude(damp)
     loss = dloss = 0.0
     for _ in range(nsteps):
         eng.tl_step(1)

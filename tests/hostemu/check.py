@@ -231,6 +231,10 @@ def check_tl(case: str, terms: int):
         _, tl = tl_directional(e, rp, rhob, np.zeros((k, rp.ncells)), da, nsteps, il)
         out["amap"] = [tl, fd.dir_amap(da, 1e-6)]
         out["amap_adjoint"] = [tl, float((ga * da).sum())]
+    # the driver amplitude drv[0] (no map: the scalar amplitude is the control)
+    if amap is None and rp.driver != P.DRIVER_NONE:
+        _, tl = tl_directional(e, rp, rhob, np.zeros((k, rp.ncells)), None, nsteps, il, damp=rp.drv[0])
+        out["amplitude"] = [tl, fd.dir_amplitude(1e-6)]
     # a direction of the static background (every field, ghosts included),
     # relative to each cell's value (the stratified background spans orders
     # of magnitude)
@@ -284,6 +288,7 @@ def check_tl_refusals():
     rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
     e = Engine(rp, w, pos)
     refused("step_before_begin", lambda: e.tl_step(1))
+    refused("amplitude_before_begin", lambda: e.tl_driver_amplitude(1.0))
     refused("map_direction_without_map",
             lambda: e.tl_begin(np.zeros((k, rp.ncells)), damap=np.ones(rp.ncells // rp.n[0])))
     inner = interior_mask(rp)

@@ -118,6 +118,39 @@ def interior_cells(rp):
     return m
 
 
+@pytest.mark.parametrize("case", ["vertical", "torsional", "spicule"])
+def test_oracle_driver_param_setter_equals_a_fresh_oracle(case):
+    """Oracle.set_driver_param (used to difference the loss in the driver
+    amplitude) acts as an oracle created with that parameter: the same
+    states bit for bit after two steps; the amplitude moves the state."""
+    if case == "vertical":
+        rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
+    elif case == "torsional":
+        rp, w, pos = torsional_case(rkon=1)
+    else:
+        rp, w, pos = spicule_case(ndim=2, rkon=1)
+    a0 = rp.drv[0]
+    o = Oracle(rp, w, pos)
+    o.set_driver_param(0, 1.25 * a0)
+    o.step(2)
+    got = o.state()
+    o.close()
+    rp.drv = [1.25 * a0] + list(rp.drv[1:])
+    o = Oracle(rp, w, pos)
+    o.step(2)
+    want = o.state()
+    o.close()
+    rp.drv = [a0] + list(rp.drv[1:])
+    o = Oracle(rp, w, pos)
+    o.step(2)
+    base = o.state()
+    o.close()
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, base)
+    with pytest.raises(ValueError):
+        Oracle(rp, w, pos).set_driver_param(99, 1.0)
+
+
 def test_spicule_acts_in_the_step_with_counter_one_only():
     rp, w, pos = spicule_case(ndim=2, rkon=0, terms=T_SOURCE)
     s = oracle_steps(rp, w, pos, 3)  # steps with counters 0, 1, 2

@@ -303,29 +303,70 @@ def test_tl_torch_forward_mode():
         e.close()
 
 
-@pytest.mark.parametrize("control", ["t_photo", "z_tr"])
+@pytest.mark.parametrize("case", ["strat2d", "strat3d", "tor", "spi"])
+def test_tl_driver_amplitude(case):
+    """The derivative in the driver amplitude drv[0] (the vertical driver on
+    2D and 3D stratified grids, the torsional and the spicule drivers), from
+    one tangent-linear run with smaug_tl_driver_amplitude, against the
+    oracle rerun with its amplitude moved by +-1e-6 and +-2e-6 relative
+    (coefficients replayed)."""
+    from smaug_amd.engine import Engine
+    from tests.test_drivers import spicule_case, torsional_case
+
+    if case == "strat2d":
+        (rp, w, pos), nsteps, il = strat_tl_case((16, 12), 2, 1), 4, 7
+    elif case == "strat3d":
+        (rp, w, pos), nsteps, il = strat_tl_case((12, 8, 8), 3, 1), 3, 6
+    elif case == "tor":
+        (rp, w, pos), nsteps, il = torsional_case(rkon=1, terms=P.TERMS_ALL), 2, 6
+    else:
+        (rp, w, pos), nsteps, il = spicule_case(ndim=2, rkon=1, terms=P.TERMS_ALL), 3, 7
+    k = P.npert(rp.ndim)
+    print(f"[tl] driver amplitude {case} {rp.n} steps={nsteps}", flush=True)
+    e = Engine(rp, w, pos)
+    fd = ReplayFD(oracle, rp, w, pos, None, nsteps, il)
+    try:
+        loss, got = tl_directional(e, rp, w[3 + 2 * rp.ndim], np.zeros((k, rp.ncells)), None, nsteps, il,
+                                   damp=rp.drv[0])
+        want = fd.dir_amplitude(EPS)
+        print(f"  A dL/dA: tangent-linear {got:.12e} fd {want:.12e}", flush=True)
+        assert abs(loss - fd.base) <= 1e-12 * abs(fd.base), (loss, fd.base)
+        assert abs(got - want) <= TOL * abs(want), (got, want)
+    finally:
+        fd.close()
+        e.close()
+
+
+@pytest.mark.parametrize("control", ["t_photo", "z_tr", "amplitude"])
 def test_tl_slice_horizon_controls(control):
     """The 2D slice configuration over its loss's own horizon: 64 (z) x 32
     (x), driven from rest, 1500 RK4 steps, L = sum over the steps of v_z^2 on
     the line z ~ 4 Mm; dL/d(control) from one tangent-linear run against the
-    oracle's replayed differences over the same 1500 steps.  Moving the
-    trajectory by one ulp moves that derivative by 1.3e-9 (T_photo) and
-    2.9e-8 (z_tr) relative, so the 1e-6 bar holds above the roundoff floor."""
+    oracle's replayed differences over the same 1500 steps, for the
+    background's controls and the driver amplitude.  Moving the trajectory
+    by one ulp moves the background controls' derivatives by 1.3e-9 (T_photo)
+    and 2.9e-8 (z_tr) relative, so the 1e-6 bar holds above the roundoff
+    floor."""
     from smaug_amd import configs
     from smaug_amd.engine import Engine
 
     rp = configs.stratified_params(ni=(64, 32), ndim=2, rkon=1)
     w, pos = configs.stratified(rp)
     k, nsteps, il = P.npert(2), 1500, 44
-    c0 = {"t_photo": 6000.0, "z_tr": 2.0e6}[control]
     print(f"[tl] slice 64x32, {nsteps} steps, control {control}", flush=True)
     e = Engine(rp, w, pos)
     fd = ReplayFD(oracle, rp, w, pos, None, nsteps, il)
     try:
-        t = configs.stratified_tangent(rp, control)
-        loss, got = tl_directional(e, rp, w[3 + 2 * rp.ndim], t[:k], None, nsteps, il, dbg=t[k:])
-        got *= c0
-        want = fd.dir_state(lambda eps: configs.stratified(rp, **{control: c0 * (1.0 + eps)})[0], EPS)
+        if control == "amplitude":  # A dL/dA
+            loss, got = tl_directional(e, rp, w[3 + 2 * rp.ndim], np.zeros((k, rp.ncells)), None, nsteps, il,
+                                       damp=rp.drv[0])
+            want = fd.dir_amplitude(EPS)
+        else:  # c0 dL/dc0
+            c0 = {"t_photo": 6000.0, "z_tr": 2.0e6}[control]
+            t = configs.stratified_tangent(rp, control)
+            loss, got = tl_directional(e, rp, w[3 + 2 * rp.ndim], t[:k], None, nsteps, il, dbg=t[k:])
+            got *= c0
+            want = fd.dir_state(lambda eps: configs.stratified(rp, **{control: c0 * (1.0 + eps)})[0], EPS)
         print(f"  {control}: tangent-linear {got:.12e} fd {want:.12e}; loss {loss:.12e} oracle {fd.base:.12e}",
               flush=True)
         assert abs(loss - fd.base) <= 1e-12 * abs(fd.base), (loss, fd.base)

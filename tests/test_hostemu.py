@@ -99,7 +99,7 @@ def transposed(pair, tol=1e-10):
     return abs(got - want) <= tol * abs(want)
 
 
-@pytest.mark.parametrize("case", ["s2r", "s2e", "s3r", "ot", "tor", "bc:2d_fixed_asymm:0"])
+@pytest.mark.parametrize("case", ["s2r", "s2e", "s3r", "ot", "tor", "spi", "bc:2d_fixed_asymm:0"])
 def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
     """The tangent-linear run (smaug_tl_*): its values reproduce the oracle's
     loss, its directional derivatives of the line loss match the replayed
@@ -108,7 +108,8 @@ def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
     a direction of the background fields and along the background's controls
     (photospheric temperature, transition height: the builder's tangent
     against the oracle rerun on the rebuilt state) it matches the
-    differences too; the engine's own state is left as it was."""
+    differences too, and for the cases without a map in the scalar driver
+    amplitude; the engine's own state is left as it was."""
     r = run(hostlib, "tl", case)
     assert abs(r["loss"][0] - r["loss"][1]) <= 1e-12 * abs(r["loss"][1]), r["loss"]
     for f, pair in enumerate(r["fields"]):
@@ -124,18 +125,22 @@ def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
         assert close(r["bg"]), r["bg"]
     for name, pair in r.get("controls", {}).items():  # photospheric T, transition height
         assert close(pair), (name, pair)
+    if "amplitude" in r:  # the scalar driver amplitude (cases without a map)
+        assert close(r["amplitude"]), r["amplitude"]
     assert r["state_untouched"]
 
 
 def test_tangent_linear_refusals_and_precision(hostlib):
     """The tangent-linear run refuses the divB source terms (-5), a step
-    before its start and a driver-map direction without a map (-1); from a
+    before its start, a driver-amplitude direction before its start and a
+    driver-map direction without a map (-1); from a
     mixed-precision engine (FP32 pairs in the stencils) its values and
     tangents stay within 1e-5 of the FP64 run's after 10 steps, and are not
     identical to them."""
     r = run(hostlib, "tl_refusals")
     assert "failed (-5)" in r["divb"], r
     assert "failed (-1)" in r["step_before_begin"], r
+    assert "failed (-1)" in r["amplitude_before_begin"], r
     assert "failed (-1)" in r["map_direction_without_map"], r
     for key in ("mixed_values_rel_l2", "mixed_tangents_rel_l2"):
         assert max(r[key]) <= 1e-5 and max(r[key]) > 0, (key, r[key])
