@@ -6,8 +6,10 @@ tensors) differentiates
     L(w0, bg) = sum_{n=1..nsteps} loss_fn(w_n, bg)
 
 in any direction of the initial evolved state w0 (np*N values, field-major,
-ghost cells included) and of the static background bg (nb*N values: e_b,
-rho_b, b_b).  loss_fn is a torch function of one state's evolved fields and
+ghost cells included), of the static background bg (nb*N values: e_b,
+rho_b, b_b) and of the driver amplitude drv[0] (a 0-d tensor whose value is
+the engine's own: the amplitude's tangent enters through
+smaug_tl_driver_amplitude; no driver amplitude map may be set).  loss_fn is a torch function of one state's evolved fields and
 of the background (flat float64 tensors on w0's device).  The
 hyperdiffusion coefficients, the wave-speed maxima and dt are held at their
 values along the trajectory, as in the reverse mode (smaug_amd.adjoint).  In
@@ -52,7 +54,7 @@ def _host(t: torch.Tensor) -> np.ndarray:
 
 
 def _run(eng: Engine, w0: np.ndarray, bg: np.ndarray, clock0, nsteps: int, loss_fn: LossFn, dev,
-         dw0=None, dbg=None) -> Tuple[torch.Tensor, torch.Tensor]:
+         dw0=None, dbg=None, damp: float = 0.0) -> Tuple[torch.Tensor, torch.Tensor]:
     """(L, dL): the forward loop from (w0, bg), and with a direction the
     tangent-linear run alongside it."""
     k = eng.npert
@@ -69,6 +71,8 @@ def _run(eng: Engine, w0: np.ndarray, bg: np.ndarray, clock0, nsteps: int, loss_
             loss = loss + loss_fn(w, bgt)
         return loss, dloss
     eng.tl_begin(dw0.reshape(k, -1), dbg=None if dbg is None else dbg.reshape(-1, eng.N))
+    if damp:
+        eng.tl_driver_amplitude(damp)
     dbgt = torch.zeros_like(bgt) if dbg is None else torch.from_numpy(dbg.reshape(-1)).to(dev)
     for _ in range(nsteps):
         eng.tl_step(1)
@@ -95,41 +99,50 @@ def _run(eng: Engine, w0: np.ndarray, bg: np.ndarray, clock0, nsteps: int, loss_
 
 class _StateLoss(torch.autograd.Function):
     @staticmethod
-    def forward(w0, bg, eng, nsteps, loss_fn, clock0):
+    def forward(w0, bg, amp, eng, nsteps, loss_fn, clock0):
         loss, _ = _run(eng, _host(w0), _host(bg), clock0, nsteps, loss_fn, w0.device)
         return loss
 
     @staticmethod
     def setup_context(ctx, inputs, output):
-        w0, bg, eng, nsteps, loss_fn, clock0 = inputs
+        w0, bg, _, eng, nsteps, loss_fn, clock0 = inputs
         ctx.eng, ctx.nsteps, ctx.loss_fn, ctx.clock0 = eng, nsteps, loss_fn, clock0
         ctx.w0, ctx.bg = w0.detach(), bg.detach()
 
     @staticmethod
-    def jvp(ctx, dw0, dbg, *_):
+    def jvp(ctx, dw0, dbg, damp, *_):
         eng = ctx.eng
         k = eng.npert
         dw = np.zeros((k, eng.N)) if dw0 is None else _host(dw0)
         db = None if dbg is None else _host(dbg)
+        da = 0.0 if damp is None else float(damp)
         _, dloss = _run(eng, _host(ctx.w0), _host(ctx.bg), ctx.clock0, ctx.nsteps, ctx.loss_fn, ctx.w0.device,
-                        dw0=dw, dbg=db)
+                        dw0=dw, dbg=db, damp=da)
         return dloss
 
 
 def state_loss(eng: Engine, w0: torch.Tensor, bg: torch.Tensor, nsteps: int, loss_fn: LossFn,
-               clock0=None) -> torch.Tensor:
-    """L(w0, bg) = sum_{n=1..nsteps} loss_fn(w_n, bg) from the initial evolved
-    state w0 (np*N float64) and background bg (nb*N float64), with
-    forward-mode derivatives in both: pass dual tensors
-    (torch.autograd.forward_ad.make_dual) and read the tangent of the result.
-    The functorch transforms (torch.func.jvp) are not supported: the engine
-    reads its inputs' storage.  clock0 = (it, qt) of w0 (default: the
-    engine's clock at entry)."""
+               clock0=None, amplitude: torch.Tensor = None) -> torch.Tensor:
+    """L(w0, bg, A) = sum_{n=1..nsteps} loss_fn(w_n, bg) from the initial
+    evolved state w0 (np*N float64) and background bg (nb*N float64), with
+    forward-mode derivatives in both and in the driver amplitude A (a 0-d
+    float64 tensor equal to the engine's drv[0]; default: that value with no
+    tangent): pass dual tensors (torch.autograd.forward_ad.make_dual) and
+    read the tangent of the result.  The functorch transforms
+    (torch.func.jvp) are not supported: the engine reads its inputs'
+    storage.  clock0 = (it, qt) of w0 (default: the engine's clock at
+    entry)."""
     if w0.dtype != torch.float64 or bg.dtype != torch.float64:
         raise TypeError("w0 and bg must be float64")
     if w0.numel() != eng.npert * eng.N or bg.numel() != (eng.nvar - eng.npert) * eng.N:
         raise ValueError("w0 needs np*N values and bg nb*N values")
+    a0 = float(eng.rp.drv[0])
+    if amplitude is None:
+        amplitude = torch.tensor(a0, dtype=torch.float64, device=w0.device)
+    elif amplitude.numel() != 1 or float(amplitude.detach()) != a0:
+        raise ValueError(f"the amplitude must be one value equal to the engine's driver amplitude {a0}")
     if clock0 is None:
         it, _, qt, _ = eng.clock()
         clock0 = (it, qt)
-    return _StateLoss.apply(w0.reshape(-1), bg.reshape(-1), eng, int(nsteps), loss_fn, tuple(clock0))
+    return _StateLoss.apply(w0.reshape(-1), bg.reshape(-1), amplitude.reshape(()), eng, int(nsteps), loss_fn,
+                            tuple(clock0))

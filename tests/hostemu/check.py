@@ -310,6 +310,44 @@ def check_tl_refusals():
     return out
 
 
+def check_tangent_torch():
+    """smaug_amd.tangent.state_loss under torch's forward-mode AD on CPU
+    tensors (the host build's device memory is host memory): value and
+    tangents in a direction of the state and background and in the driver
+    amplitude, against the oracle's loss and replayed differences."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+
+    from smaug_amd.tangent import line_vz2_loss_bg, state_loss
+
+    rp, w, pos = strat_case(ni=(16, 12), ndim=2, rkon=1)
+    k, il, nsteps = P.npert(2), 7, 3
+    rng = np.random.default_rng(5)
+    inner = interior_mask(rp)
+    scale = np.sqrt((w[:k, inner] ** 2).mean(axis=1))
+    d = rng.standard_normal((k, rp.ncells)) * scale[:, None] * inner[None, :]
+    dbg = rng.standard_normal(w[k:].shape) * np.abs(w[k:])
+    w0, bg = torch.tensor(w[:k].ravel()), torch.tensor(w[k:].ravel())
+    a0 = torch.tensor(rp.drv[0], dtype=torch.float64)
+    e = Engine(rp, w, pos)
+    fd = ReplayFD(oracle, rp, w, pos, None, nsteps, il)
+    fn = line_vz2_loss_bg(e, il)
+    with fwAD.dual_level():
+        val, tan = fwAD.unpack_dual(state_loss(e, fwAD.make_dual(w0, torch.tensor(d.ravel())),
+                                               fwAD.make_dual(bg, torch.tensor(dbg.ravel())), nsteps, fn,
+                                               clock0=(rp.it0, rp.qt0), amplitude=fwAD.make_dual(a0, a0.clone())))
+    out = {"loss": [float(val), fd.base],
+           "all": [float(tan), fd.dir_w0(d, 1e-6) + fd.dir_bg(dbg, 1e-6) + fd.dir_amplitude(1e-6)]}
+    try:
+        state_loss(e, w0, bg, nsteps, fn, amplitude=1.5 * a0)
+        out["wrong_amplitude"] = "accepted"
+    except ValueError as err:
+        out["wrong_amplitude"] = str(err)
+    e.close()
+    fd.close()
+    return out
+
+
 def check_autograd(rkon: int):
     """smaug_amd.adjoint.driver_loss on CPU tensors (the host build's device
     memory is host memory); torch's stream calls are replaced by no-ops."""
@@ -482,6 +520,8 @@ def main(argv):
         r = check_adj(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
     elif what == "tl_refusals":
         r = check_tl_refusals()
+    elif what == "tl_torch":
+        r = check_tangent_torch()
     elif what == "tl":
         r = check_tl(argv[1], int(argv[2], 0) if len(argv) > 2 else P.TERMS_ALL)
     else:

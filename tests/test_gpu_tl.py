@@ -264,7 +264,8 @@ def test_tl_torch_forward_mode():
     """smaug_amd.tangent.state_loss under torch's forward-mode AD (dual
     tensors on the GPU, the loss in torch): the value equals the oracle's
     loss and the tangent its replayed differences, in a direction of the
-    initial state and of the background together, and of the state alone."""
+    initial state and of the background together, of the state alone and
+    of the driver amplitude."""
     import torch.autograd.forward_ad as fwAD
 
     from smaug_amd.engine import Engine
@@ -298,6 +299,17 @@ def test_tl_torch_forward_mode():
             tan = fwAD.unpack_dual(state_loss(e, a, bg, nsteps, fn, clock0=(rp.it0, rp.qt0))).tangent
         want = fd.dir_w0(d, EPS)
         assert abs(float(tan) - want) <= TOL * abs(want), (float(tan), want)
+        # the driver amplitude as a dual input: tangent A gives A dL/dA
+        a0 = torch.tensor(rp.drv[0], dtype=torch.float64, device=dev)
+        with fwAD.dual_level():
+            amp = fwAD.make_dual(a0, a0.clone())
+            tan = fwAD.unpack_dual(state_loss(e, w0, bg, nsteps, fn, clock0=(rp.it0, rp.qt0),
+                                              amplitude=amp)).tangent
+        want = fd.dir_amplitude(EPS)
+        print(f"  amplitude: forward mode {float(tan):.12e} fd {want:.12e}", flush=True)
+        assert abs(float(tan) - want) <= TOL * abs(want), (float(tan), want)
+        with pytest.raises(ValueError):  # the engine's amplitude is fixed at creation
+            state_loss(e, w0, bg, nsteps, fn, amplitude=1.5 * a0)
     finally:
         fd.close()
         e.close()

@@ -146,6 +146,18 @@ def test_tangent_linear_refusals_and_precision(hostlib):
         assert max(r[key]) <= 1e-5 and max(r[key]) > 0, (key, r[key])
 
 
+def test_tangent_torch_forward_mode(hostlib):
+    """The forward-mode torch binding (smaug_amd.tangent.state_loss) on CPU
+    tensors: the loss equals the oracle's; one dual run in a direction of
+    the state, the background and the driver amplitude together matches the
+    sum of the replayed differences (1e-6); an amplitude other than the
+    engine's is refused."""
+    r = run(hostlib, "tl_torch")
+    assert abs(r["loss"][0] - r["loss"][1]) <= 1e-12 * abs(r["loss"][1]), r["loss"]
+    assert close(r["all"]), r["all"]
+    assert "driver amplitude" in r["wrong_amplitude"], r
+
+
 BC_CASES = ["bc:2d_symm_cont:1", "bc:2d_fixed_asymm:0", "bc:3d_cont_symm_fixed:1", "bc:3d_asymm_fixed_cont:1"]
 
 
