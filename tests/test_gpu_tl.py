@@ -11,7 +11,10 @@ The values of the run reproduce the oracle's loss (1e-12) and the FP64
 forward step's state (1e-13 per variable), and the run leaves the engine's
 own state untouched.  The driver amplitude map stays off here: it belongs
 to the differentiable entry points gated off on the GPU (SMAUG_ADJOINT), so
-its direction is checked in the host build (tests/test_hostemu.py).
+its direction is checked in the host build (tests/test_hostemu.py).  The
+driver's scalar amplitude drv[0] is a direction here
+(smaug_tl_driver_amplitude), against the oracle rerun with its amplitude
+moved.
 """
 from __future__ import annotations
 
