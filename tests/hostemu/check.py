@@ -289,6 +289,19 @@ def check_tl_refusals():
     e = Engine(rp, w, pos)
     refused("step_before_begin", lambda: e.tl_step(1))
     refused("amplitude_before_begin", lambda: e.tl_driver_amplitude(1.0))
+    e.tl_begin(np.zeros((k, rp.ncells)))
+    refused("amplitude_not_finite", lambda: e.tl_driver_amplitude(float("nan")))
+    e.set_driver_map(np.full(rp.ncells // rp.n[0], 200.0))
+    e.tl_begin(np.zeros((k, rp.ncells)))
+    refused("amplitude_with_map", lambda: e.tl_driver_amplitude(1.0))
+    e.tl_end()
+    e.set_driver_map(None)
+    ro, wo, po = ot_case(ni=16, rkon=1)  # no driver
+    eo = Engine(ro, wo, po)
+    eo.tl_begin(np.zeros((P.npert(2), ro.ncells)))
+    refused("amplitude_without_driver", lambda: eo.tl_driver_amplitude(1.0))
+    eo.tl_end()
+    eo.close()
     refused("map_direction_without_map",
             lambda: e.tl_begin(np.zeros((k, rp.ncells)), damap=np.ones(rp.ncells // rp.n[0])))
     inner = interior_mask(rp)

@@ -132,15 +132,17 @@ def test_tangent_linear_matches_differences_and_adjoint(hostlib, case):
 
 def test_tangent_linear_refusals_and_precision(hostlib):
     """The tangent-linear run refuses the divB source terms (-5), a step
-    before its start, a driver-amplitude direction before its start and a
-    driver-map direction without a map (-1); from a
+    before its start, a driver-amplitude direction before its start, not
+    finite, with a driver map or without a driver, and a driver-map
+    direction without a map (-1); from a
     mixed-precision engine (FP32 pairs in the stencils) its values and
     tangents stay within 1e-5 of the FP64 run's after 10 steps, and are not
     identical to them."""
     r = run(hostlib, "tl_refusals")
     assert "failed (-5)" in r["divb"], r
     assert "failed (-1)" in r["step_before_begin"], r
-    assert "failed (-1)" in r["amplitude_before_begin"], r
+    for key in ("amplitude_before_begin", "amplitude_not_finite", "amplitude_with_map", "amplitude_without_driver"):
+        assert "failed (-1)" in r[key], (key, r)
     assert "failed (-1)" in r["map_direction_without_map"], r
     for key in ("mixed_values_rel_l2", "mixed_tangents_rel_l2"):
         assert max(r[key]) <= 1e-5 and max(r[key]) > 0, (key, r[key])
